@@ -1,0 +1,222 @@
+/*
+ * negf_amd.h -- C ABI of the MI355X-native NEGF charge path.
+ *
+ * Drop-in boundary for the reference ELEQTRONeX NEGF material path
+ * (AMReX-Microelectronics/Electrostatic).  Plain pointers and sizes only;
+ * every entry point returns an int status (NEGF_OK on success) and
+ * negf_last_error() describes the last failure of the calling thread.
+ * Device pointers (d_*) are hipMalloc'd memory owned by the caller;
+ * stream arguments are hipStream_t passed as void* (NULL = default stream).
+ *
+ * The reference has no C ABI of its own: its boundary is the C++ class
+ * c_Nanostructure_Base (Source/Solver/Transport/Nanostructure.H:15-45) over
+ * c_NEGF_Common<T> (Source/Solver/Transport/NEGF/NEGF_Common.H:478-605).  Each
+ * declaration below names the reference member it replaces.
+ */
+#ifndef NEGF_AMD_H_
+#define NEGF_AMD_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } negf_complex;
+
+/* Status codes.  The reference aborts (amrex::Abort) where these are returned. */
+#define NEGF_OK 0
+#define NEGF_EARG 1      /* invalid argument                                   */
+#define NEGF_EALLOC 2    /* device allocation failed                           */
+#define NEGF_EROOT 3     /* no surface-GF root with Im<0 (CNT.cpp:379-382)      */
+#define NEGF_ENONCONV 4  /* Broyden_Step > Broyden_threshold_maxstep            */
+                         /* (Transport.cpp:389-393)                             */
+#define NEGF_ENAN 5      /* non-finite value                                   */
+#define NEGF_EDEVICE 6   /* HIP runtime error                                  */
+#define NEGF_ENOTSUP 7   /* configuration outside what this build supports     */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *negf_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int negf_version(void);
+/* Largest chain length (sites) the segmented RGF kernels accept. */
+int negf_max_sites(void);
+
+/* ------------------------------------------------------------------------
+ * Level 1: batched RGF over an energy list for one scalar tridiagonal chain.
+ * Replaces the per-energy loop of c_NEGF_Common::Compute_RhoEq / Compute_Rho0
+ * / Compute_GR_atPoles (NEGF_Common.cpp:3630-3759, 4002-4089, 3833-3952):
+ * host RGF sweep (:3667-3714) + per-site kernel (:3742-3754).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    int N;                    /* sites                                        */
+    double t_even, t_odd;     /* |hopping| on links (n,n+1), n even / n odd:  */
+                              /* Hb = Hc = -t (CNT.cpp:322-335)               */
+    double Uc[2];             /* contact potentials (U_contact)               */
+    int use_decimation;       /* 0: closed-form lead GF, 1: decimation        */
+    int decimation_layers, decimation_max_iter, decimation_rel_error;
+} negf_chain_params;
+
+/* acc[n] = sum_k Re(coef_k * G_nn(E_k)) for n < N (overwritten).
+ * d_diagG (optional, [nE][N]) receives G_nn(E_k).  Leads: Sigma = t_odd^2 g
+ * with g the closed-form surface GF of the (t_even, t_odd) chain. */
+int negf_rgf_diag_accumulate(const negf_chain_params *p, const double *d_U,
+                             int nE, const negf_complex *d_E,
+                             const negf_complex *d_coef, double *d_acc,
+                             negf_complex *d_diagG, void *stream);
+
+/* Non-equilibrium form (Compute_RhoNonEq kernel NEGF_Common.cpp:3377-3497):
+ * acc[n] = sum_k coefL_k Gamma_L |G_n0|^2 + coefR_k Gamma_R |G_n,N-1|^2.
+ * d_integrand (optional, [nE][3]) receives the same integrand with
+ * coefficients iL_k, iR_k at sites N/2, N/4, 3N/4 (:3457-3495); pass NULL
+ * d_iL/d_iR to reuse coefL/coefR. */
+int negf_rgf_noneq_accumulate(const negf_chain_params *p, const double *d_U,
+                              int nE, const negf_complex *d_E,
+                              const double *d_coefL, const double *d_coefR,
+                              const double *d_iL, const double *d_iR,
+                              double *d_acc, double *d_integrand,
+                              void *stream);
+
+/* Per-energy trace of the spectral function and transmission
+ * (Compute_DensityOfStates NEGF_Common.cpp:2640-2777): for each E_k,
+ * dos[k] = sum_n (Gamma_L |G_n0|^2 + Gamma_R |G_n,N-1|^2),
+ * trans[k] = Gamma_L Gamma_R |G_0,N-1|^2 (device arrays of nE). */
+int negf_rgf_dos(const negf_chain_params *p, const double *d_U, int nE,
+                 const negf_complex *d_E, double *d_dos, double *d_trans,
+                 void *stream);
+
+/* ------------------------------------------------------------------------
+ * Level 2: NEGF material (one zigzag CNT in mode space).  Mirrors
+ * c_NEGF_Common<T> + c_CNT (CNT.H:13-103) with the reference's input keys.
+ * ------------------------------------------------------------------------ */
+typedef struct negf_tube negf_tube;
+
+typedef struct {
+    /* CNT_default.* / <name>.* (CNT.cpp:13-21) */
+    int type_id[2];                 /* (m, n); zigzag n = 0               */
+    double acc;                     /* C-C bond length (m)                 */
+    double gamma;                   /* hopping (eV)                        */
+    int num_modes;                  /* BLOCK_SIZE: lowest distinct modes   */
+    /* NS_default.* / <name>.* (NEGF_Common.cpp:288-696) */
+    int num_unitcells;
+    double E_valence_min, E_pole_max, E_zPlus_imag;
+    double Fermi_tail_factors[2];
+    int eq_integration_pts[3];
+    int num_noneq_paths;
+    int noneq_integration_pts[8];
+    double noneq_percent_intercuts[8];
+    double contact_mu[2];           /* contact_mu (eV)                     */
+    double contact_T[2];            /* contact_T (K)                       */
+    int use_decimation, decimation_layers, decimation_max_iter;
+    int decimation_rel_error;       /* int, as in NEGF_Common.H:475        */
+    int flag_adaptive_integration_limits;
+    int integrand_correction_interval;
+    double kT_window_around_singularity[2];
+    int flatband_dos_integration_pts;
+} negf_tube_params;
+
+/* Fill p with the reference defaults (NEGF_Common.H:384-476, CNT.H:17-18). */
+void negf_tube_params_default(negf_tube_params *p);
+
+/* Initialize_NEGF_Params + Initialize_NEGF (NEGF_Common.cpp:1256-1306):
+ * modes, Hamiltonian, contacts, energy limits, paths, Rho0. */
+int negf_tube_create(const negf_tube_params *p, negf_tube **out);
+int negf_tube_destroy(negf_tube *t);
+/* Hsize_glo (get_Hsize CNT.H:74). */
+int negf_tube_num_sites(const negf_tube *t);
+/* h_PTD_glo_vec: axial ring positions in nm (NEGF_Common.cpp:1074-1080). */
+int negf_tube_positions(const negf_tube *t, double *ptd_nm);
+/* Selected modes: beta and degeneracy (Define_SortedModeVector CNT.cpp:227-286). */
+int negf_tube_modes(const negf_tube *t, int *mode_id, double *beta, int *degen);
+/* Potential at sites and contacts: U_n = -<phi>, U_contact = U at end sites
+ * (Obtain_PotentialAtSites Nanostructure.cpp:494-504). Host arrays. */
+int negf_tube_set_potential(negf_tube *t, const double *U, const double Uc[2]);
+/* set_Contact_Electrochemical_Potential (NEGF_Common.cpp:356-368) and
+ * contact temperatures; limits and paths are rebuilt at the next solve. */
+int negf_tube_set_contacts(negf_tube *t, const double mu[2], const double T[2]);
+/* Compute_Rho0 (NEGF_Common.cpp:3957-4092); Im Rho0 per site to host. */
+int negf_compute_rho0(negf_tube *t, double *rho0_im);
+/* Solve_NEGF (NEGF_Common.cpp:1309-1340) -> Compute_InducedCharge
+ * (:2850-2900): Q_n = Im RhoEq + Re RhoNonEq - Im Rho0 into Q (host, N).
+ * iter is the SCF iteration (adaptive limits every
+ * integrand_correction_interval).  *n_intg_pts = get_Total_Integration_Pts. */
+int negf_solve(negf_tube *t, int iter, double *Q, int *n_intg_pts);
+/* Same, with Q left in device memory d_Q (N doubles). */
+int negf_solve_device(negf_tube *t, int iter, double *d_Q, int *n_intg_pts,
+                      void *stream);
+/* Charge components of the last solve (_chargeComp.dat, :3000-3023):
+ * Im RhoEq, Re RhoNonEq, Im Rho0 (host arrays of N). */
+int negf_get_charge_components(const negf_tube *t, double *eq_im,
+                               double *noneq_re, double *rho0_im);
+/* Energy limits / path summary after the last update:
+ * out[0..9] = mu_min, mu_max, kT_min, kT_max, flag_noneq, num_poles,
+ * E_contour_right.re, E_zeta.im, E_eta.re, E_rightmost.re. */
+int negf_tube_limits(const negf_tube *t, double *out);
+/* Non-equilibrium integration layout after the last update: npaths, and
+ * per path points and percent intercuts. */
+int negf_tube_noneq_layout(const negf_tube *t, int *npaths, int *pts,
+                           double *intercuts);
+/* Compute_DensityOfStates over nE real energies (E_zPlus added):
+ * DOS = spin * sum_modes degen * trace / (2 pi) / atoms_per_unitcell and
+ * T = sum_modes degen * Gamma_L Gamma_R |G_0,N-1|^2 (host arrays). */
+int negf_tube_dos(negf_tube *t, int nE, const double *E_real, double *dos,
+                  double *trans);
+
+/* ------------------------------------------------------------------------
+ * Broyden modified-second on device (Execute_Broyden_Modified_Second_
+ * Algorithm_Parallel Broyden_Second_Parallel.cpp:213-486; CPU ordering
+ * :9-211; Set_/Reset_Broyden_Parallel Broyden_Parallel_General.cpp:68-355).
+ * ------------------------------------------------------------------------ */
+typedef struct negf_broyden negf_broyden;
+/* norm_type: 0 absolute, 1 relative (Broyden_norm_type). */
+int negf_broyden_create(int S, double fraction, int max_step, int norm_type,
+                        negf_broyden **out);
+int negf_broyden_destroy(negf_broyden *b);
+int negf_broyden_set_nin(negf_broyden *b, const double *n_in);
+int negf_broyden_get_nin(const negf_broyden *b, double *n_in);
+/* One update from n_out (host array of S).  Returns NEGF_ENONCONV when the
+ * history is full.  max_norm = Broyden_Norm, l2 = Broyden_NormSum_Curr;
+ * norm_per_site (optional) = h_Norm. */
+int negf_broyden_step(negf_broyden *b, const double *n_out, double *max_norm,
+                      double *l2, double *norm_per_site);
+/* Device form: n_out and the updated n_in stay on the device. */
+int negf_broyden_step_device(negf_broyden *b, const double *d_n_out,
+                             double *max_norm, double *l2, void *stream);
+const double *negf_broyden_device_nin(const negf_broyden *b);
+int negf_broyden_reset(negf_broyden *b, int keep_n_in, double init_value);
+int negf_broyden_get_step(const negf_broyden *b);
+
+/* ------------------------------------------------------------------------
+ * SCF driver (c_TransportSolver::Solve Transport.cpp:359-539) over one tube
+ * with a dense 1/r-with-image electrostatic stand-in for the MLMG solve and
+ * gather: U_i = -V_gate - sum_j K_ij n_in_j.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    double eps_r, R_ring_nm, R_image_nm;   /* Coulomb stand-in            */
+    double Broyden_fraction, Broyden_max_norm;
+    int Broyden_norm_type, Broyden_threshold_maxstep;
+    int reset_with_previous_charge_distribution;
+    double NS_initial_deposit_value;
+} negf_scf_params;
+
+typedef struct negf_scf negf_scf;
+int negf_scf_create(negf_tube *t, const negf_scf_params *p, negf_scf **out);
+int negf_scf_destroy(negf_scf *s);
+/* One bias point: iterate until Broyden_Norm <= Broyden_max_norm, then
+ * Reset_ForNextBiasStep.  U_out, Q_out: host arrays of N. */
+int negf_scf_bias_point(negf_scf *s, double V_gate, double *U_out,
+                        double *Q_out, int *iters, double *final_norm);
+/* Dense kernel K (N x N, row-major, V per e) used by the stand-in. */
+int negf_scf_kernel(const negf_scf *s, double *K);
+
+/* ------------------------------------------------------------------------
+ * Output writers (NEGF_Common.cpp:1343-1418, 3044-3112, 4223-4254):
+ * <prefix>_Qout.dat, <prefix>_U.dat, <prefix>_norm.dat in the reference's
+ * column formats.
+ * ------------------------------------------------------------------------ */
+int negf_write_step_files(const char *prefix, int N, const double *ptd_nm,
+                          const double *Q, const double *U,
+                          const double *norm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

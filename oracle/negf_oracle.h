@@ -1,0 +1,157 @@
+/*
+ * negf_oracle.h -- CPU oracle for the NEGF charge path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C++ restatement of the
+ * reference ELEQTRONeX NEGF charge path (AMReX-Microelectronics/Electrostatic),
+ * written to check the HIP product against.  Only tests/, the smoke() entry
+ * and the CPU-baseline leg of bench.py may load it.  The product library
+ * (electrostatic_amd/) never links, loads or calls anything in oracle/.
+ *
+ * Every function cites the reference file:line it restates.  The reference
+ * needs AMReX (not vendored, not installable here), so it cannot be built;
+ * the restatement keeps the reference's loop order (host-serial RGF per
+ * energy, O(N^2) contact walks, Broyden in the CPU order).
+ *
+ * Complex arithmetic: amrex::GpuComplex is not available; this file uses its
+ * own complex struct written from the definitions (multiply = textbook,
+ * divide = a*conj(b)/|b|^2, sqrt = principal branch via std::sqrt).  Parity
+ * with the reference is therefore unpinned at the level of a few ulp per
+ * complex operation.
+ */
+#ifndef NEGF_ORACLE_H_
+#define NEGF_ORACLE_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } orc_c;
+
+#define ORC_MAX_MODES 32
+#define ORC_MAX_PATHS 8
+#define ORC_MAX_POLES 256
+
+/* One tube (mode-space tight-binding chain, independent diagonal modes). */
+typedef struct {
+    int N;                       /* sites (rings)                              */
+    int nmodes;                  /* BLOCK_SIZE: independent diagonal modes     */
+    double beta[ORC_MAX_MODES];  /* per-mode beta (link n even)                */
+    int degen[ORC_MAX_MODES];    /* per-mode degeneracy                        */
+    double gamma;                /* hopping on odd links; also lead tau        */
+    int spin_degen;              /* 2                                          */
+    /* contour parameters (NEGF_Common.H:384-405) */
+    double E_valence_min, E_pole_max, E_zPlus_imag;
+    double tail_lower, tail_upper;
+    int eq_pts[3];
+    int num_noneq_paths;
+    int noneq_pts[ORC_MAX_PATHS];
+    double noneq_intercuts[ORC_MAX_PATHS]; /* percent, num_noneq_paths-1      */
+    double mu[2];                /* contact electrochemical potentials (eV)    */
+    double T[2];                 /* contact temperatures (K)                   */
+    /* decimation (NEGF_Common.H:469-476) */
+    int use_decimation, decimation_layers, decimation_max_iter;
+    int decimation_rel_error;    /* int in the reference (NEGF_Common.H:475)   */
+    /* adaptive integration limits (NEGF_Common.H:453-460) */
+    double kT_window[2];
+} orc_tube;
+
+typedef struct {
+    double kT_contact[2], kT_min, kT_max, mu_min, mu_max;
+    int flag_noneq;
+    orc_c E_contour_left, E_contour_right, E_rightmost, E_zeta, E_eta, E_zPlus;
+    int num_poles;               /* poles enclosed (0 if non-equilibrium)      */
+    int num_poles_formula;       /* int((E_pole_max-pi kT)/(2 pi kT)+1)        */
+    orc_c poles[ORC_MAX_POLES];
+} orc_limits;
+
+/* ---- quadrature and paths ---- */
+void orc_gauss_legendre(int n, double *x, double *w);
+void orc_gl_path(orc_c a, orc_c b, int n, int type, orc_c *E, double *w,
+                 orc_c *mul);
+
+/* ---- CNT mode space ---- */
+int orc_cnt_sorted_modes(int m, int n, double acc, double gamma, int *mode,
+                         double *gap, int *degen);
+double orc_cnt_beta(int J, int m, double gamma);
+void orc_cnt_ring_positions(int m, int num_unitcells, double acc,
+                            double *ptd_nm);
+
+/* ---- lead surface GF, Fermi function ---- */
+int orc_surface_gf(orc_c E, double U, double beta, double gamma, orc_c *g);
+orc_c orc_decimation_gf(orc_c EmU, double beta, double gamma, int layers,
+                        int max_iter, int rel_error);
+orc_c orc_fermi(orc_c z, double kT);
+
+/* ---- energy limits ---- */
+void orc_energy_limits(const orc_tube *t, const double mu[2], orc_limits *L);
+
+/* ---- RGF (host, serial) ---- */
+void orc_rgf(int N, const orc_c *alpha, const orc_c Hb[2], const orc_c Hc[2],
+             orc_c *Xt, orc_c *X, orc_c *Yt, orc_c *Y);
+
+/* ---- charge components (per site, complex; summed over modes) ---- */
+int orc_rho0(const orc_tube *t, orc_c *Rho0);
+int orc_rho_eq(const orc_tube *t, const double *U, const double Uc[2],
+               orc_c *RhoEq, int *npts);
+int orc_rho_noneq(const orc_tube *t, const double *U, const double Uc[2],
+                  orc_c *RhoNonEq, double *integrand, int *npts);
+int orc_solve_negf(const orc_tube *t, const double *U, const double Uc[2],
+                   const orc_c *Rho0, double *Q, double *comps, int *npts);
+int orc_adaptive_intercuts(orc_tube *t, const double *U, const double Uc[2]);
+
+/* ---- generic energy-list accumulation (batched-RGF checker) ---- */
+int orc_accumulate_diagG(int N, const double *U, double t_even, double t_odd,
+                         const double Uc[2], double lead_beta,
+                         double lead_gamma, int nE, const orc_c *E,
+                         const orc_c *coef, double *acc_re, orc_c *diagG);
+int orc_accumulate_noneq(int N, const double *U, double t_even, double t_odd,
+                         const double Uc[2], double lead_beta,
+                         double lead_gamma, int nE, const orc_c *E,
+                         const double *coefL, const double *coefR,
+                         double *acc_re);
+
+/* ---- DOS / transmission (post-process) ---- */
+int orc_dos_transmission(const orc_tube *t, const double *U,
+                         const double Uc[2], int nE, const orc_c *E,
+                         int atoms_per_unitcell, double *dos, double *trans,
+                         double *ldos);
+
+/* ---- Broyden modified-second ---- */
+typedef struct orc_broyden orc_broyden;
+orc_broyden *orc_broyden_create(int S, double fraction, int max_step,
+                                int norm_type);
+void orc_broyden_free(orc_broyden *b);
+void orc_broyden_set_nin(orc_broyden *b, const double *n_in);
+void orc_broyden_get_nin(const orc_broyden *b, double *n_in);
+int orc_broyden_step(orc_broyden *b, const double *n_out, double *max_norm,
+                     double *l2, double *norm_per_site);
+void orc_broyden_reset(orc_broyden *b, int keep_n_in, double init_value);
+int orc_broyden_get_step(const orc_broyden *b);
+
+/* ---- electrostatic stand-in and SCF loop ---- */
+typedef struct {
+    double eps_r;        /* relative permittivity                         */
+    double R_ring_nm;    /* ring radius regularisation                    */
+    double R_image_nm;   /* image distance                                */
+} orc_coulomb;
+void orc_coulomb_kernel(int N, const double *ptd_nm, const orc_coulomb *c,
+                        double *K);
+
+typedef struct {
+    double fraction;     /* Broyden_fraction                               */
+    double max_norm;     /* Broyden_max_norm                               */
+    int norm_type;       /* 0 absolute, 1 relative                         */
+    int max_step;        /* Broyden_threshold_maxstep                      */
+    int reset_with_previous;
+    double initial_value;
+} orc_scf_params;
+
+int orc_scf_bias_point(const orc_tube *t, const orc_c *Rho0, const double *K,
+                       double V_gate, const orc_scf_params *sp,
+                       orc_broyden *b, double *U_out, double *Q_out,
+                       int *iters, double *final_norm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

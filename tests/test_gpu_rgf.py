@@ -1,0 +1,162 @@
+"""GPU parity of the batched RGF entry points (negf_rgf_*) against the oracle.
+
+Tolerances: site charge 1e-10 relative (max-norm over sites, relative to
+max |Q|); diag G^R 1e-9 relative per element where |G| > 1e-6 max|G|.
+"""
+import numpy as np
+import pytest
+
+from electrostatic_amd import _lib
+from oracle import oracle as O
+from tests.gpu_util import (chain_params, dev_array, dev_zeros, ptr, to_np,
+                            smooth_potential)
+
+pytestmark = pytest.mark.gpu
+
+BETA17 = 2.2286917788826917   # 2*2.5*cos(6 pi/17): (17,0) lowest mode
+GAMMA = 2.5
+
+
+def energies(n_arc, n_real, seed=0):
+    """Arc-contour energies (large Im) plus real-axis energies (eta=1e-5)."""
+    rng = np.random.default_rng(seed)
+    arc = []
+    if n_arc:
+        th = np.sort(rng.uniform(0.05, np.pi - 0.05, n_arc))
+        arc = -4.5 + 5.5 * np.exp(1j * th)
+    real = np.sort(rng.uniform(-3, 3, n_real)) + 1e-5j if n_real else []
+    return np.concatenate([np.asarray(arc, complex), np.asarray(real, complex)])
+
+
+def run_diag(U, te, to, Uc, E, coef, want_G=True, **dec):
+    lib = _lib.load()
+    N = U.size
+    p = chain_params(N, te, to, Uc, **dec)
+    dU = dev_array(U)
+    dE = dev_array(E)
+    dc = dev_array(coef)
+    acc = dev_zeros(N)
+    G = dev_zeros(E.size * N, complex_=True) if want_G else None
+    rc = lib.negf_rgf_diag_accumulate(p, ptr(dU), E.size, ptr(dE), ptr(dc),
+                                      ptr(acc), ptr(G), None)
+    _lib.check(rc)
+    out = to_np(acc)
+    g = to_np(G, True).reshape(E.size, N) if want_G else None
+    return out, g
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("N", [2, 5, 16, 40, 257, 400, 1001, 2048, 4096])
+def test_diag_accumulate_matches_oracle(N):
+    U = smooth_potential(N, seed=N)
+    E = energies(40, 40, seed=N)
+    rng = np.random.default_rng(N + 1)
+    coef = rng.normal(size=E.size) + 1j * rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    acc_ref, G_ref, st = O.accumulate_diagG(U, BETA17, GAMMA, Uc, BETA17, GAMMA,
+                                            E, coef, want_diag=True)
+    assert st == 0
+    acc, G = run_diag(U, BETA17, GAMMA, Uc, E, coef)
+    assert rel_err(acc, acc_ref) < 1e-10
+    big = np.abs(G_ref) > 1e-6 * np.abs(G_ref).max()
+    assert (np.abs(G - G_ref)[big] / np.abs(G_ref)[big]).max() < 1e-9
+
+
+def test_diag_config2_shape():
+    """Batched-RGF workload shape: N=4096, contour arc + real axis."""
+    N = 4096
+    U = smooth_potential(N, seed=2)
+    te, to = 2.5 * np.cos(np.pi / 17), 2.5
+    E = energies(512, 512, seed=3)
+    coef = np.exp(1j * np.linspace(0, 3, E.size))
+    Uc = (U[0], U[-1])
+    acc_ref, st = O.accumulate_diagG(U, te, to, Uc, te, to, E, coef)
+    acc, _ = run_diag(U, te, to, Uc, E, coef, want_G=False)
+    assert rel_err(acc, acc_ref) < 1e-10
+
+
+def test_diag_decimation_matches_oracle():
+    N = 64
+    U = smooth_potential(N, seed=5)
+    E = energies(10, 10, seed=5)
+    coef = np.ones(E.size, complex)
+    Uc = (U[0], U[-1])
+    t = O.make_tube(N, [BETA17], [1], use_decimation=1, decimation_layers=10,
+                    decimation_max_iter=30, decimation_rel_error=0)
+    # oracle: alpha with decimated Sigma via the generic per-energy path
+    G_ref = []
+    for e in E:
+        g0 = O.decimation_gf(e - Uc[0], BETA17, GAMMA, 10, 30, 0)
+        g1 = O.decimation_gf(e - Uc[1], BETA17, GAMMA, 10, 30, 0)
+        H = np.zeros((N, N), complex)
+        for k in range(N - 1):
+            H[k, k + 1] = H[k + 1, k] = BETA17 if k % 2 == 0 else GAMMA
+        A = e * np.eye(N) - H - np.diag(U)
+        A[0, 0] -= GAMMA ** 2 * g0
+        A[-1, -1] -= GAMMA ** 2 * g1
+        G_ref.append(np.diag(np.linalg.inv(A)))
+    G_ref = np.array(G_ref)
+    del t
+    acc, G = run_diag(U, BETA17, GAMMA, Uc, E, coef, use_decimation=1,
+                      layers=10, max_iter=30, rel_error=0)
+    assert np.abs(G - G_ref).max() / np.abs(G_ref).max() < 1e-10
+
+
+@pytest.mark.parametrize("N", [3, 64, 400, 999])
+def test_noneq_accumulate_matches_oracle_walk(N):
+    lib = _lib.load()
+    U = smooth_potential(N, seed=7 + N, lo=-0.3, hi=0.2)
+    E = np.linspace(-2.5, 1.5, 60) + 1e-5j
+    rng = np.random.default_rng(N)
+    cL = rng.normal(size=E.size)
+    cR = rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    ref, st = O.accumulate_noneq(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E, cL, cR)
+    p = chain_params(N, BETA17, GAMMA, Uc)
+    dU, dE = dev_array(U), dev_array(E)
+    dL, dR = dev_array(cL), dev_array(cR)
+    acc = dev_zeros(N)
+    integ = dev_zeros(3 * E.size)
+    _lib.check(lib.negf_rgf_noneq_accumulate(p, ptr(dU), E.size, ptr(dE), ptr(dL),
+                                             ptr(dR), None, None, ptr(acc),
+                                             ptr(integ), None))
+    assert rel_err(to_np(acc), ref) < 1e-10
+    # integrand samples at N/2, N/4, 3N/4: A_nn for one energy at a time
+    got = to_np(integ).reshape(E.size, 3)
+    for k in (0, 17, 59):
+        for w, n in enumerate((N // 2, N // 4, N * 3 // 4)):
+            one_L = np.zeros(E.size)
+            one_R = np.zeros(E.size)
+            one_L[k] = cL[k]
+            one_R[k] = cR[k]
+            r, _ = O.accumulate_noneq(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E,
+                                      one_L, one_R)
+            assert abs(got[k, w] - r[n]) <= 1e-10 * max(abs(r[n]), 1e-300) + 1e-300
+
+
+def test_dos_transmission_matches_oracle():
+    lib = _lib.load()
+    N = 120
+    U = smooth_potential(N, seed=11, lo=-0.2, hi=0.2)
+    E = np.linspace(-2, 2, 81) + 1e-5j
+    Uc = (U[0], U[-1])
+    t = O.make_tube(N, [BETA17], [1], spin=1)
+    dos_ref, tr_ref, st = O.dos_transmission(t, U, np.array(Uc), E, 1)
+    p = chain_params(N, BETA17, GAMMA, Uc)
+    dU, dE = dev_array(U), dev_array(E)
+    dd, dt = dev_zeros(E.size), dev_zeros(E.size)
+    _lib.check(lib.negf_rgf_dos(p, ptr(dU), E.size, ptr(dE), ptr(dd), ptr(dt), None))
+    dos = to_np(dd) / (2 * np.pi)
+    assert rel_err(dos, dos_ref) < 1e-10
+    assert rel_err(to_np(dt), tr_ref) < 1e-10
+
+
+def test_bad_arguments_raise():
+    lib = _lib.load()
+    p = chain_params(1, BETA17, GAMMA, (0, 0))
+    rc = lib.negf_rgf_diag_accumulate(p, None, 0, None, None, None, None, None)
+    assert rc == 1
+    assert b"invalid" in lib.negf_last_error()
