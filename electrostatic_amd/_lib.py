@@ -83,6 +83,13 @@ SIGNATURES = {
     "negf_rgf_diag_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp]),
     "negf_rgf_noneq_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]),
     "negf_rgf_dos": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp]),
+    "negf_rgf_plan_create": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, C.c_int, C.POINTER(vp)]),
+    "negf_rgf_plan_execute": (C.c_int, [vp, vp, vp, vp]),
+    "negf_rgf_plan_status": (C.c_int, [vp, vp]),
+    "negf_rgf_plan_enable_timing": (C.c_int, [vp, C.c_int]),
+    "negf_rgf_plan_kernel_time": (C.c_int, [vp, dp, ip]),
+    "negf_rgf_plan_destroy": (C.c_int, [vp]),
+    "negf_gl_path": (C.c_int, [negf_complex, negf_complex, C.c_int, C.c_int, vp, dp, vp]),
     "negf_tube_params_default": (None, [C.POINTER(negf_tube_params)]),
     "negf_tube_create": (C.c_int, [C.POINTER(negf_tube_params), C.POINTER(vp)]),
     "negf_tube_destroy": (C.c_int, [vp]),

@@ -11,7 +11,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libnegf_amd.so")
 
-HIP_SOURCES = ["negf_engine.hip", "negf_tube.hip", "broyden.hip", "scf.hip"]
+
+INCLUDED = {"rgf_wave.hip"}
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -33,10 +34,21 @@ def hipcc():
     return "hipcc"
 
 
+def sources():
+    """Every translation unit of the library: csrc/*.hip and csrc/*.cpp
+    (kernel files listed in INCLUDED are #included by negf_engine.hip)."""
+    out = []
+    for name in sorted(os.listdir(CSRC)):
+        if name in INCLUDED:
+            continue
+        if name.endswith(".hip") or name.endswith(".cpp"):
+            out.append(os.path.join(CSRC, name))
+    return out
+
+
 def build(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES
-            if os.path.exists(os.path.join(CSRC, s))]
+    srcs = sources()
     tmp = LIB + ".tmp"
     cmd = [hipcc()] + HIPCC_FLAGS + ["-o", tmp] + srcs
     if verbose:

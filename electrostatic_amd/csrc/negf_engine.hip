@@ -9,7 +9,7 @@
 #include <mutex>
 
 #include "negf_internal.h"
-#include "rgf_seg.hip"
+#include "rgf_wave.hip"
 
 namespace negf {
 
@@ -69,54 +69,53 @@ void HostBuf::release()
     cap = 0;
 }
 
-// ---------------------------------------------------------------- variants
-struct VariantInfo {
-    int K, NS;
-};
-static const VariantInfo kVariants[] = {{16, 4}, {32, 8}};
-static const int kNumVariants = 2;
-
-int choose_variant(int N, int* variant, int* Lf)
+// ---------------------------------------------------------------- segmentation
+int choose_segments(int N, int* P, int* L)
 {
-    int L0 = (N + NEGF_SEGMENTS - 1) / NEGF_SEGMENTS;
-    for (int v = 0; v < kNumVariants; ++v) {
-        int K = kVariants[v].K;
-        int L = (L0 + K - 1) / K * K;
-        if (L <= K * kVariants[v].NS) {
-            *variant = v;
-            *Lf = L;
+    for (int p = 1; p <= 64; p <<= 1) {
+        int l0 = (N + p - 1) / p;
+        int l = (l0 + NEGF_SUB_K - 1) / NEGF_SUB_K * NEGF_SUB_K;
+        if (l <= NEGF_SEG_MAX) {
+            *P = p;
+            *L = l;
             return 1;
         }
     }
     return 0;
 }
-int max_sites() { return NEGF_SEGMENTS * kVariants[kNumVariants - 1].K * kVariants[kNumVariants - 1].NS; }
+int max_sites() { return NEGF_MAX_SITES; }
 
-template <int K, int NS>
-static hipError_t launch_kernel(int kind, bool wg, int nblocks, int lds, const LaunchParams& lp, hipStream_t st)
+template <int KIND, bool WG>
+static void launch_sweep(int nblk, const WLaunch& lp, hipStream_t st)
 {
-    dim3 grid(nblocks), block(NEGF_BLOCK_THREADS);
-    if (kind == KIND_EQ) {
-        if (wg)
-            hipLaunchKernelGGL((rgf_seg_kernel<K, NS, KIND_EQ, true>), grid, block, lds, st, lp);
-        else
-            hipLaunchKernelGGL((rgf_seg_kernel<K, NS, KIND_EQ, false>), grid, block, lds, st, lp);
-    } else if (kind == KIND_NONEQ) {
-        hipLaunchKernelGGL((rgf_seg_kernel<K, NS, KIND_NONEQ, false>), grid, block, lds, st, lp);
-    } else {
-        hipLaunchKernelGGL((rgf_seg_kernel<K, NS, KIND_DOS, false>), grid, block, 0, st, lp);
-    }
-    return hipGetLastError();
+    hipLaunchKernelGGL((sweep_kernel<NEGF_SUB_K, NEGF_SUB_NS, KIND, WG>), dim3(nblk), dim3(256), 0, st, lp);
 }
 
 // ---------------------------------------------------------------- plan
 RgfPlan::~RgfPlan()
 {
-    DevBuf* bufs[] = {&d_tubes_,   &d_bchain0_,       &d_bnchain_,    &d_btube_,       &d_bpart_,
-                      &d_Ut_,      &d_part_,          &d_tmp_,        &d_status_,      &d_red_src_off_,
-                      &d_red_nrows_, &d_red_N_,       &d_red_dst_off_, &d_red2_src_off_, &d_red2_nrows_,
-                      &d_red2_dst_off_};
+    DevBuf* bufs[] = {&d_tubes_,       &d_item_tube_,   &d_chain_tube_,   &d_sblk_tube_,    &d_sblk_chain_,
+                      &d_Tm_,          &d_St_,          &d_Dd_,           &d_part_,         &d_tmp_,
+                      &d_dos_part_,    &d_status_,      &d_red_src_off_,  &d_red_nrows_,    &d_red_N_,
+                      &d_red_dst_off_, &d_red2_src_off_, &d_red2_nrows_,  &d_red2_dst_off_};
     for (DevBuf* b : bufs) b->release();
+    for (hipEvent_t e : ev_start_) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_stop_) (void)hipEventDestroy(e);
+}
+
+int RgfPlan::kernel_time(double* total_ms, int* launches)
+{
+    double tot = 0.0;
+    for (size_t i = 0; i < ev_used_; ++i) {
+        NEGF_HIP_TRY(hipEventSynchronize(ev_stop_[i]));
+        float ms = 0.f;
+        NEGF_HIP_TRY(hipEventElapsedTime(&ms, ev_start_[i], ev_stop_[i]));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int)ev_used_;
+    ev_used_ = 0;
+    return NEGF_OK;
 }
 
 template <class T>
@@ -128,11 +127,11 @@ static int upload(DevBuf& b, const std::vector<T>& v)
     return NEGF_OK;
 }
 
-int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, const ChainDesc* d_chains,
+int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, ChainDesc* d_chains, int nchains,
                    const DecParams& dec)
 {
-    if (tubes.empty()) {
-        set_error("RgfPlan::build: no tubes");
+    if (tubes.empty() || !d_chains || nchains <= 0) {
+        set_error("RgfPlan::build: no tubes or chains");
         return NEGF_EARG;
     }
     if (dec.use && (dec.layers < 2 || dec.layers > NEGF_MAX_DECIMATION_LAYERS)) {
@@ -142,109 +141,83 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, co
     kind_ = kind;
     write_G_ = write_G;
     d_chains_ = d_chains;
+    nchains_ = nchains;
     dec_ = dec;
     tubes_ = tubes;
     ntubes_ = (int)tubes.size();
-    Lf_.assign(ntubes_, 0);
-    u_off_.assign(ntubes_, 0);
-    variant_ = -1;
-    long long uoff = 0;
+    std::vector<WTube> wt(ntubes_);
+    std::vector<int> item_tube, chain_tube(nchains, 0), sblk_tube, sblk_chain;
+    std::vector<long long> rsrc, rdst, tdst, t2src;
+    std::vector<int> rnrows, rN, t2rows;
+    long long poff = 0, toff = 0;
+    int items = 0;
     max_N_ = 0;
-    lds_bytes_ = 0;
-    any_global_atomic_ = false;
+    max_groups_ = 0;
     for (int t = 0; t < ntubes_; ++t) {
         const RgfTube& tb = tubes[t];
-        if (tb.N < 2 || !tb.dU || tb.nchain < 0 || tb.t_odd <= 0) {
-            set_error("RgfPlan::build: invalid tube %d (N=%d)", t, tb.N);
+        if (tb.N < 2 || !tb.dU || tb.nchain <= 0 || tb.t_odd <= 0 || tb.chain0 < 0 ||
+            tb.chain0 + tb.nchain > nchains) {
+            set_error("RgfPlan::build: invalid tube %d (N=%d, chains %d+%d of %d)", t, tb.N, tb.chain0, tb.nchain,
+                      nchains);
             return NEGF_EARG;
         }
-        int v, L;
-        if (!choose_variant(tb.N, &v, &L)) {
+        int P, L;
+        if (!choose_segments(tb.N, &P, &L)) {
             set_error("chain of %d sites exceeds the supported maximum %d", tb.N, max_sites());
             return NEGF_ENOTSUP;
         }
-        variant_ = std::max(variant_, v);
-        Lf_[t] = L;
-        max_N_ = std::max(max_N_, tb.N);
-    }
-    // one variant per launch: recompute Lf for the common (largest) variant
-    for (int t = 0; t < ntubes_; ++t) {
-        int K = kVariants[variant_].K;
-        int L0 = (tubes[t].N + NEGF_SEGMENTS - 1) / NEGF_SEGMENTS;
-        Lf_[t] = (L0 + K - 1) / K * K;
-        u_off_[t] = uoff;
-        uoff += (long long)Lf_[t] * NEGF_SEGMENTS;
-        if (tubes[t].N <= NEGF_LDS_SITES_MAX)
-            lds_bytes_ = std::max(lds_bytes_, tubes[t].N * (int)sizeof(double));
-        else
-            any_global_atomic_ = true;
-    }
-    int rc = d_Ut_.ensure(uoff * sizeof(double));
-    if (rc) return rc;
-
-    // tube descriptors
-    std::vector<TubeDesc> td(ntubes_);
-    for (int t = 0; t < ntubes_; ++t) {
-        const RgfTube& tb = tubes[t];
-        TubeDesc& d = td[t];
-        std::memset(&d, 0, sizeof(d));
-        d.N = tb.N;
-        d.seg_len = Lf_[t];
-        d.nsub = Lf_[t] / kVariants[variant_].K;
-        d.lds_ok = tb.N <= NEGF_LDS_SITES_MAX;
-        d.t_odd = tb.t_odd;
-        d.Uc0 = tb.Uc0;
-        d.Uc1 = tb.Uc1;
-        d.u_off = u_off_[t];
-        d.out_off = tb.out_off;
-        for (int q = 0; q < 3; ++q) d.sample[q] = tb.sample[q];
-    }
-    // blocks
-    std::vector<int> bchain0, bnchain, btube;
-    std::vector<long long> bpart;
-    std::vector<long long> rsrc, rdst;
-    std::vector<int> rnrows, rN;
-    long long poff = 0;
-    max_rows_ = 0;
-    for (int t = 0; t < ntubes_; ++t) {
-        const RgfTube& tb = tubes[t];
-        int nb = (tb.nchain + NEGF_CHAINS_PER_BLOCK - 1) / NEGF_CHAINS_PER_BLOCK;
-        rsrc.push_back(poff);
-        rnrows.push_back(nb);
-        rN.push_back(tb.N);
-        for (int b = 0; b < nb; ++b) {
-            bchain0.push_back(tb.chain0 + b * NEGF_CHAINS_PER_BLOCK);
-            bnchain.push_back(std::min(NEGF_CHAINS_PER_BLOCK, tb.nchain - b * NEGF_CHAINS_PER_BLOCK));
-            btube.push_back(t);
-            bpart.push_back(poff + (long long)b * tb.N);
+        WTube& w = wt[t];
+        std::memset(&w, 0, sizeof(w));
+        w.N = tb.N;
+        w.L = L;
+        w.P = P;
+        w.nsub = L / NEGF_SUB_K;
+        w.t_odd = tb.t_odd;
+        w.Uc0 = tb.Uc0;
+        w.Uc1 = tb.Uc1;
+        w.U = tb.dU;
+        w.chain0 = tb.chain0;
+        w.nchain = tb.nchain;
+        w.item0 = items;
+        w.part_off = poff;
+        for (int q = 0; q < 3; ++q) w.sample[q] = tb.sample[q];
+        const int G = (tb.nchain + 63) / 64;
+        for (int i = 0; i < G * P; ++i) item_tube.push_back(t);
+        items += G * P;
+        for (int c = 0; c < tb.nchain; ++c) chain_tube[tb.chain0 + c] = t;
+        const int cpb = 256 / P;  // chains per scan block (padding lanes of the last group included)
+        for (int c0 = 0; c0 < G * 64; c0 += cpb) {
+            sblk_tube.push_back(t);
+            sblk_chain.push_back(c0);
         }
-        poff += (long long)nb * tb.N;
-        max_rows_ = std::max(max_rows_, nb);
-    }
-    nblocks_ = (int)bchain0.size();
-    part_elems_ = poff;
-    if ((rc = d_part_.ensure(std::max<long long>(1, poff) * sizeof(double)))) return rc;
-    // reduction pass 1: chunks of reduce_chunk_ rows -> tmp; pass 2: tmp -> out
-    reduce_chunk_ = 16;
-    max_chunks_ = (max_rows_ + reduce_chunk_ - 1) / reduce_chunk_;
-    long long toff = 0;
-    std::vector<long long> tdst, t2src;
-    std::vector<int> t2rows;
-    for (int t = 0; t < ntubes_; ++t) {
-        int nch = (rnrows[t] + reduce_chunk_ - 1) / reduce_chunk_;
+        rsrc.push_back(poff);
+        rnrows.push_back(G);
+        rN.push_back(tb.N);
+        poff += (long long)G * tb.N;
+        int nch = (G + reduce_chunk_ - 1) / reduce_chunk_;
         tdst.push_back(toff);
         t2src.push_back(toff);
         t2rows.push_back(nch);
-        toff += (long long)nch * rN[t];
-        rdst.push_back(tubes[t].out_off);
+        toff += (long long)nch * tb.N;
+        rdst.push_back(tb.out_off);
+        max_N_ = std::max(max_N_, tb.N);
+        max_groups_ = std::max(max_groups_, G);
     }
-    tmp_elems_ = toff;
+    nitems_ = items;
+    nsblk_ = (int)sblk_tube.size();
+    max_chunks_ = (max_groups_ + reduce_chunk_ - 1) / reduce_chunk_;
+    int rc;
+    if ((rc = d_Tm_.ensure((size_t)items * 9 * 64 * sizeof(double)))) return rc;
+    if ((rc = d_St_.ensure((size_t)items * 10 * 64 * sizeof(double)))) return rc;
+    if ((rc = d_Dd_.ensure((size_t)nchains * 3 * sizeof(double)))) return rc;
+    if ((rc = d_part_.ensure(std::max<long long>(1, poff) * sizeof(double)))) return rc;
     if ((rc = d_tmp_.ensure(std::max<long long>(1, toff) * sizeof(double)))) return rc;
-    if ((rc = upload(d_tubes_, td))) return rc;
-    if ((rc = upload(d_bchain0_, bchain0))) return rc;
-    if ((rc = upload(d_bnchain_, bnchain))) return rc;
-    if ((rc = upload(d_btube_, btube))) return rc;
-    if ((rc = upload(d_bpart_, bpart))) return rc;
+    if (kind == KIND_DOS && (rc = d_dos_part_.ensure((size_t)items * 64 * sizeof(double)))) return rc;
+    if ((rc = upload(d_tubes_, wt))) return rc;
+    if ((rc = upload(d_item_tube_, item_tube))) return rc;
+    if ((rc = upload(d_chain_tube_, chain_tube))) return rc;
+    if ((rc = upload(d_sblk_tube_, sblk_tube))) return rc;
+    if ((rc = upload(d_sblk_chain_, sblk_chain))) return rc;
     if ((rc = upload(d_red_src_off_, rsrc))) return rc;
     if ((rc = upload(d_red_nrows_, rnrows))) return rc;
     if ((rc = upload(d_red_N_, rN))) return rc;
@@ -259,7 +232,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, co
 
 int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st)
 {
-    if (variant_ < 0 || nblocks_ == 0) {
+    if (nitems_ == 0) {
         set_error("RgfPlan::enqueue: plan not built");
         return NEGF_EARG;
     }
@@ -267,64 +240,78 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         set_error("diag G output requested without a buffer");
         return NEGF_EARG;
     }
-    if ((kind_ == KIND_DOS) && !d_chain_out) {
+    if (kind_ == KIND_DOS && !d_chain_out) {
         set_error("DOS output buffer missing");
         return NEGF_EARG;
     }
-    // transposed potentials
-    for (int t = 0; t < ntubes_; ++t) {
-        int total = Lf_[t] * NEGF_SEGMENTS;
-        hipLaunchKernelGGL(transpose_u_kernel, dim3((total + 255) / 256), dim3(256), 0, st, tubes_[t].dU,
-                           tubes_[t].N, Lf_[t], d_Ut_.as<double>() + u_off_[t]);
-    }
-    if (any_global_atomic_ && kind_ != KIND_DOS) {
-        hipLaunchKernelGGL(fill_kernel, dim3((part_elems_ + 255) / 256), dim3(256), 0, st, d_part_.as<double>(),
-                           part_elems_, 0.0);
-    }
-    NEGF_HIP_TRY(hipGetLastError());
-    LaunchParams lp;
-    lp.chains = const_cast<ChainDesc*>(d_chains_);
-    lp.tubes = d_tubes_.as<TubeDesc>();
-    lp.block_chain0 = d_bchain0_.as<int>();
-    lp.block_nchain = d_bnchain_.as<int>();
-    lp.block_tube = d_btube_.as<int>();
-    lp.block_part = d_bpart_.as<long long>();
-    lp.Ut = d_Ut_.as<double>();
-    lp.part = d_part_.as<double>();
-    lp.diagG = d_diagG;
-    lp.chain_out = d_chain_out;
-    lp.status = d_status_.as<int>();
-    lp.use_decimation = dec_.use;
-    lp.dec_layers = dec_.layers;
-    lp.dec_max_iter = dec_.max_iter;
-    lp.dec_rel_error = dec_.rel_error;
-    hipLaunchKernelGGL(sigma_kernel, dim3(nblocks_), dim3(64), 0, st, lp);
-    NEGF_HIP_TRY(hipGetLastError());
-    hipError_t e;
-    if (variant_ == 0)
-        e = launch_kernel<16, 4>(kind_, write_G_, nblocks_, lds_bytes_, lp, st);
-    else
-        e = launch_kernel<32, 8>(kind_, write_G_, nblocks_, lds_bytes_, lp, st);
-    if (e != hipSuccess) {
-        set_error("rgf kernel launch failed: %s", hipGetErrorString(e));
-        return NEGF_EDEVICE;
-    }
-    if (kind_ == KIND_DOS) return NEGF_OK;
-    if (!d_out) {
+    if (kind_ != KIND_DOS && !d_out) {
         set_error("output buffer missing");
         return NEGF_EARG;
     }
-    // fixed-order reduction of the partial rows
-    {
-        dim3 g1((max_N_ + 255) / 256, max_chunks_, ntubes_);
-        hipLaunchKernelGGL(reduce_rows_kernel, g1, dim3(256), 0, st, d_part_.as<double>(),
-                           d_red_src_off_.as<long long>(), d_red_nrows_.as<int>(), d_red_N_.as<int>(),
-                           reduce_chunk_, d_tmp_.as<double>(), d_red_dst_off_.as<long long>());
-        dim3 g2((max_N_ + 255) / 256, 1, ntubes_);
-        hipLaunchKernelGGL(reduce_rows_kernel, g2, dim3(256), 0, st, d_tmp_.as<double>(),
-                           d_red2_src_off_.as<long long>(), d_red2_nrows_.as<int>(), d_red_N_.as<int>(),
-                           1 << 30, d_out, d_red2_dst_off_.as<long long>());
+    WLaunch lp;
+    std::memset(&lp, 0, sizeof(lp));
+    lp.chains = d_chains_;
+    lp.tubes = d_tubes_.as<WTube>();
+    lp.item_tube = d_item_tube_.as<int>();
+    lp.nitems = nitems_;
+    lp.Tm = d_Tm_.as<double>();
+    lp.St = d_St_.as<double>();
+    lp.Dd = d_Dd_.as<double>();
+    lp.part = d_part_.as<double>();
+    lp.diagG = d_diagG;
+    lp.chain_out = d_chain_out;
+    lp.dos_part = d_dos_part_.as<double>();
+    lp.sblk_tube = d_sblk_tube_.as<int>();
+    lp.sblk_chain = d_sblk_chain_.as<int>();
+    lp.status = d_status_.as<int>();
+    lp.kind = kind_;
+
+    hipLaunchKernelGGL(sigma_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, d_chains_,
+                       d_chain_tube_.as<int>(), nchains_, d_tubes_.as<WTube>(), dec_.use, dec_.layers,
+                       dec_.max_iter, dec_.rel_error, d_status_.as<int>());
+    NEGF_HIP_TRY(hipGetLastError());
+    if (timing_) {
+        if (ev_used_ == ev_start_.size()) {
+            hipEvent_t a, b;
+            NEGF_HIP_TRY(hipEventCreate(&a));
+            NEGF_HIP_TRY(hipEventCreate(&b));
+            ev_start_.push_back(a);
+            ev_stop_.push_back(b);
+        }
+        NEGF_HIP_TRY(hipEventRecord(ev_start_[ev_used_], st));
     }
+    const int nblk = (nitems_ + 3) / 4;
+    hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3(nblk), dim3(256), 0, st, lp);
+    hipLaunchKernelGGL(scan_kernel, dim3(nsblk_), dim3(256), 0, st, lp);
+    if (kind_ == KIND_EQ) {
+        if (write_G_)
+            launch_sweep<KIND_EQ, true>(nblk, lp, st);
+        else
+            launch_sweep<KIND_EQ, false>(nblk, lp, st);
+    } else if (kind_ == KIND_NONEQ) {
+        launch_sweep<KIND_NONEQ, false>(nblk, lp, st);
+    } else {
+        launch_sweep<KIND_DOS, false>(nblk, lp, st);
+    }
+    NEGF_HIP_TRY(hipGetLastError());
+    if (timing_) {
+        NEGF_HIP_TRY(hipEventRecord(ev_stop_[ev_used_], st));
+        ++ev_used_;
+    }
+    if (kind_ == KIND_DOS) {
+        hipLaunchKernelGGL(dos_finish_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, lp, ntubes_);
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
+    // fixed-order reduction of the group rows
+    dim3 g1((max_N_ + 255) / 256, max_chunks_, ntubes_);
+    hipLaunchKernelGGL(reduce_rows_kernel, g1, dim3(256), 0, st, d_part_.as<double>(),
+                       d_red_src_off_.as<long long>(), d_red_nrows_.as<int>(), d_red_N_.as<int>(), reduce_chunk_,
+                       d_tmp_.as<double>(), d_red_dst_off_.as<long long>());
+    dim3 g2((max_N_ + 255) / 256, 1, ntubes_);
+    hipLaunchKernelGGL(reduce_rows_kernel, g2, dim3(256), 0, st, d_tmp_.as<double>(),
+                       d_red2_src_off_.as<long long>(), d_red2_nrows_.as<int>(), d_red_N_.as<int>(), 1 << 30,
+                       d_out, d_red2_dst_off_.as<long long>());
     NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }
@@ -475,7 +462,7 @@ int negf_rgf_diag_accumulate(const negf_chain_params* p, const double* d_U, int 
                        (const double2*)d_coef, p->t_even, c.chains.as<ChainDesc>());
     NEGF_HIP_TRY(hipGetLastError());
     std::vector<RgfTube> tubes{one_tube(p, d_U, nE)};
-    if ((rc = c.plan.build(KIND_EQ, d_diagG != nullptr, tubes, c.chains.as<ChainDesc>(), dec_of(p)))) return rc;
+    if ((rc = c.plan.build(KIND_EQ, d_diagG != nullptr, tubes, c.chains.as<ChainDesc>(), nE, dec_of(p)))) return rc;
     if ((rc = c.plan.enqueue(d_acc, (double2*)d_diagG, nullptr, st))) return rc;
     return finish_status(c.plan, st);
 }
@@ -501,7 +488,7 @@ int negf_rgf_noneq_accumulate(const negf_chain_params* p, const double* d_U, int
     std::vector<RgfTube> tubes{one_tube(p, d_U, nE)};
     if (!d_integrand)
         for (int q = 0; q < 3; ++q) tubes[0].sample[q] = -1;
-    if ((rc = c.plan.build(KIND_NONEQ, false, tubes, c.chains.as<ChainDesc>(), dec_of(p)))) return rc;
+    if ((rc = c.plan.build(KIND_NONEQ, false, tubes, c.chains.as<ChainDesc>(), nE, dec_of(p)))) return rc;
     if ((rc = c.plan.enqueue(d_acc, nullptr, d_integrand, st))) return rc;
     return finish_status(c.plan, st);
 }
@@ -525,12 +512,110 @@ int negf_rgf_dos(const negf_chain_params* p, const double* d_U, int nE, const ne
                        (const double2*)nullptr, p->t_even, c.chains.as<ChainDesc>());
     NEGF_HIP_TRY(hipGetLastError());
     std::vector<RgfTube> tubes{one_tube(p, d_U, nE)};
-    if ((rc = c.plan.build(KIND_DOS, false, tubes, c.chains.as<ChainDesc>(), dec_of(p)))) return rc;
+    if ((rc = c.plan.build(KIND_DOS, false, tubes, c.chains.as<ChainDesc>(), nE, dec_of(p)))) return rc;
     if ((rc = c.plan.enqueue(nullptr, nullptr, c.pairs.as<double>(), st))) return rc;
     hipLaunchKernelGGL(split_dos_kernel, dim3((nE + 255) / 256), dim3(256), 0, st, nE, c.pairs.as<double>(), d_dos,
                        d_trans);
     NEGF_HIP_TRY(hipGetLastError());
     return finish_status(c.plan, st);
+}
+
+struct negf_rgf_plan {
+    negf::RgfPlan plan;
+    negf::DevBuf chains;
+    int nE = 0, N = 0;
+    bool want_G = false;
+};
+
+int negf_rgf_plan_create(const negf_chain_params* p, const double* d_U, int nE, const negf_complex* d_E,
+                         const negf_complex* d_coef, int want_diagG, negf_rgf_plan** out)
+{
+    clear_error();
+    if (!out) {
+        set_error("out is NULL");
+        return NEGF_EARG;
+    }
+    int rc = check_params(p, d_U, nE, d_E);
+    if (rc) return rc;
+    negf_rgf_plan* h = new negf_rgf_plan;
+    h->nE = nE;
+    h->N = p->N;
+    h->want_G = want_diagG != 0;
+    if ((rc = h->chains.ensure(sizeof(ChainDesc) * nE))) {
+        delete h;
+        return rc;
+    }
+    hipLaunchKernelGGL(pack_eq_chains, dim3((nE + 255) / 256), dim3(256), 0, 0, nE, (const double2*)d_E,
+                       (const double2*)d_coef, p->t_even, h->chains.as<ChainDesc>());
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        set_error("pack_eq_chains failed: %s", hipGetErrorString(e));
+        delete h;
+        return NEGF_EDEVICE;
+    }
+    std::vector<RgfTube> tubes{one_tube(p, d_U, nE)};
+    if ((rc = h->plan.build(KIND_EQ, h->want_G, tubes, h->chains.as<ChainDesc>(), nE, dec_of(p)))) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return NEGF_OK;
+}
+
+int negf_rgf_plan_execute(negf_rgf_plan* h, double* d_acc, negf_complex* d_diagG, void* stream)
+{
+    clear_error();
+    if (!h || !d_acc || (h->want_G && !d_diagG)) {
+        set_error("invalid plan execute arguments");
+        return NEGF_EARG;
+    }
+    return h->plan.enqueue(d_acc, (double2*)d_diagG, nullptr, (hipStream_t)stream);
+}
+
+int negf_rgf_plan_status(negf_rgf_plan* h, void* stream)
+{
+    clear_error();
+    if (!h) {
+        set_error("plan is NULL");
+        return NEGF_EARG;
+    }
+    return finish_status(h->plan, (hipStream_t)stream);
+}
+
+int negf_rgf_plan_enable_timing(negf_rgf_plan* h, int on)
+{
+    if (!h) return NEGF_EARG;
+    h->plan.enable_timing(on != 0);
+    return NEGF_OK;
+}
+
+int negf_rgf_plan_kernel_time(negf_rgf_plan* h, double* total_ms, int* launches)
+{
+    clear_error();
+    if (!h || !total_ms || !launches) {
+        set_error("invalid arguments");
+        return NEGF_EARG;
+    }
+    return h->plan.kernel_time(total_ms, launches);
+}
+
+int negf_rgf_plan_destroy(negf_rgf_plan* h)
+{
+    if (!h) return NEGF_EARG;
+    (void)hipDeviceSynchronize();
+    delete h;
+    return NEGF_OK;
+}
+
+int negf_gl_path(negf_complex a, negf_complex b, int n, int type, negf_complex* E, double* w, negf_complex* mul)
+{
+    clear_error();
+    if (n <= 0 || !E || !w || !mul || (type != 0 && type != 1)) {
+        set_error("invalid gl path arguments");
+        return NEGF_EARG;
+    }
+    gl_path(a.re, a.im, b.re, b.im, n, type, (double*)E, w, (double*)mul);
+    return NEGF_OK;
 }
 
 }  // extern "C"

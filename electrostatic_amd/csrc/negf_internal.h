@@ -59,48 +59,52 @@ struct RgfTube {
     long long out_off = 0;        // offset of this tube's N outputs in d_out
 };
 
-// A reusable plan: block tables and workspace sized once, enqueued many times.
+// A reusable plan: tables and workspace sized once, enqueued many times.
 class RgfPlan {
    public:
     ~RgfPlan();
-    // kind: KIND_EQ / KIND_NONEQ / KIND_DOS.  d_chains: device ChainDesc array.
-    int build(int kind, bool write_G, const std::vector<RgfTube>& tubes, const ChainDesc* d_chains,
+    // kind: KIND_EQ / KIND_NONEQ / KIND_DOS.  d_chains: device ChainDesc array
+    // (chains of tube t are [tubes[t].chain0, +nchain)); Sigma is written into it.
+    int build(int kind, bool write_G, const std::vector<RgfTube>& tubes, ChainDesc* d_chains, int nchains,
               const DecParams& dec);
-    // Enqueue: potential transpose, RGF kernel(s), fixed-order row reduction.
-    // d_out: per-tube outputs at out_off (KIND_EQ/NONEQ); d_diagG: [chain][N];
-    // d_chain_out: per-chain outputs (integrand samples / DOS pairs).
+    // Enqueue: lead self-energies, transfer matrices, segment scan, sweeps and
+    // the fixed-order row reduction.  d_out: per-tube outputs at out_off
+    // (KIND_EQ/NONEQ); d_diagG: [chain][N]; d_chain_out: per-chain outputs
+    // (integrand samples [slot][3], DOS/T pairs [slot][2]).
     int enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st);
     // Read and clear the device status word (synchronises the stream).
     int check_status(hipStream_t st, int* status);
-    int num_blocks() const { return nblocks_; }
-    int variant() const { return variant_; }
+    int num_items() const { return nitems_; }
+    // hipEvent timing of the RGF kernels (transfer + scan + sweep)
+    void enable_timing(bool on) { timing_ = on; }
+    int kernel_time(double* total_ms, int* launches);
 
    private:
     int kind_ = 0;
     bool write_G_ = false;
-    int variant_ = 0;
-    int nblocks_ = 0;
-    int ntubes_ = 0;
-    int lds_bytes_ = 0;
-    bool any_global_atomic_ = false;
-    const ChainDesc* d_chains_ = nullptr;
+    int nitems_ = 0, nchains_ = 0, ntubes_ = 0, nsblk_ = 0;
+    int max_N_ = 0, max_groups_ = 0, max_chunks_ = 0;
+    int reduce_chunk_ = 16;
+    ChainDesc* d_chains_ = nullptr;
     DecParams dec_;
     std::vector<RgfTube> tubes_;
-    std::vector<int> Lf_;
-    std::vector<long long> u_off_;
-    long long part_elems_ = 0;
-    int reduce_chunk_ = 16;
-    int max_rows_ = 0, max_N_ = 0, max_chunks_ = 0;
-    long long tmp_elems_ = 0;
-    DevBuf d_tubes_, d_bchain0_, d_bnchain_, d_btube_, d_bpart_;
-    DevBuf d_Ut_, d_part_, d_tmp_, d_status_;
+    DevBuf d_tubes_, d_item_tube_, d_chain_tube_, d_sblk_tube_, d_sblk_chain_;
+    DevBuf d_Tm_, d_St_, d_Dd_, d_part_, d_tmp_, d_dos_part_, d_status_;
     DevBuf d_red_src_off_, d_red_nrows_, d_red_N_, d_red_dst_off_;   // pass 1
     DevBuf d_red2_src_off_, d_red2_nrows_, d_red2_dst_off_;          // pass 2
+    bool timing_ = false;
+    std::vector<hipEvent_t> ev_start_, ev_stop_;
+    size_t ev_used_ = 0;
 };
 
-// Choose the kernel variant for a chain of N sites: returns the full segment
-// length Lf (multiple of the variant's sub-block), or 0 if N is too long.
-int choose_variant(int N, int* variant, int* Lf);
+// Host-side quadrature (Quadrature::Gauss_Legendre CodeUtil.cpp:640-684 and
+// c_IntegrationPath::Define_GaussLegendrePoints IntegrationPath.cpp:41-147).
+void gauss_legendre(int n, double* x, double* w);
+void gl_path(double ar, double ai, double br, double bi, int n, int type, double* E, double* w, double* mul);
+
+// Segmentation of a chain of N sites: P segments (power of two <= 64) of L
+// sites (multiple of NEGF_SUB_K, at most NEGF_SEG_MAX); 0 if N is too long.
+int choose_segments(int N, int* P, int* L);
 int max_sites();
 
 }  // namespace negf

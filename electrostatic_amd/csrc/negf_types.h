@@ -2,13 +2,13 @@
 // and the gfx950 kernels.
 #pragma once
 
-#define NEGF_SEGMENTS 16              // segments per chain = one DPP row of 16 lanes
-#define NEGF_CHAINS_PER_WAVE 4        // 64 lanes / 16 segments
 #define NEGF_WAVES_PER_BLOCK 8
-#define NEGF_CHAINS_PER_BLOCK (NEGF_CHAINS_PER_WAVE * NEGF_WAVES_PER_BLOCK)
 #define NEGF_BLOCK_THREADS 512
+#define NEGF_SUB_K 16                 // sites per sub-block
+#define NEGF_SUB_NS 8                 // sub-blocks per segment (max)
+#define NEGF_SEG_MAX (NEGF_SUB_K * NEGF_SUB_NS)  // sites per segment (max)
 #define NEGF_MAX_DECIMATION_LAYERS 128
-#define NEGF_LDS_SITES_MAX 9216       // per-block LDS charge accumulator capacity (72 KiB)
+#define NEGF_MAX_SITES (64 * NEGF_SEG_MAX)      // 64 segments of 128 sites
 
 namespace negf {
 
@@ -38,10 +38,10 @@ struct TubeDesc {
     int N;
     int seg_len;         // full segment length Lf (multiple of the sub-block K)
     int nsub;            // sub-blocks per full segment
-    int lds_ok;          // N <= NEGF_LDS_SITES_MAX
+    int pad0;
     double t_odd;        // hopping on odd links (gamma); also lead tau
     double Uc0, Uc1;     // contact potentials
-    long long u_off;     // offset of the transposed potential in the U table
+    const double* U;     // device potential, N values
     long long out_off;   // offset of this tube's rows in the partial buffer
     int sample[3];       // integrand sample sites (N/2, N/4, 3N/4)
     int pad;
@@ -54,7 +54,7 @@ struct LaunchParams {
     const int* block_nchain;   // chains in each block (<= 16)
     const int* block_tube;     // tube of each block
     const long long* block_part; // row of the partial buffer written by each block
-    const double* Ut;          // transposed potentials
+    const double* Ut;          // unused (kept for layout stability)
     double* part;              // [rows][N] per-block partial sums
     double2* diagG;            // optional [chain][N]
     double* chain_out;         // optional per-chain outputs

@@ -83,6 +83,33 @@ int negf_rgf_dos(const negf_chain_params *p, const double *d_U, int nE,
                  const negf_complex *d_E, double *d_dos, double *d_trans,
                  void *stream);
 
+/* Reusable plan for the diagonal form: chain descriptors and block tables are
+ * built once; each execute enqueues the lead self-energies, the potential
+ * transpose, the RGF kernel and the fixed-order reduction on the stream
+ * without host synchronisation.  The production form of
+ * negf_rgf_diag_accumulate for loops that call it repeatedly. */
+typedef struct negf_rgf_plan negf_rgf_plan;
+int negf_rgf_plan_create(const negf_chain_params *p, const double *d_U,
+                         int nE, const negf_complex *d_E,
+                         const negf_complex *d_coef, int want_diagG,
+                         negf_rgf_plan **out);
+int negf_rgf_plan_execute(negf_rgf_plan *plan, double *d_acc,
+                          negf_complex *d_diagG, void *stream);
+/* Synchronise the stream and report NEGF_EROOT if any energy had no
+ * surface-GF root with Im<0 since the last check. */
+int negf_rgf_plan_status(negf_rgf_plan *plan, void *stream);
+/* hipEvent timing of the RGF kernel alone (on the execute stream). */
+int negf_rgf_plan_enable_timing(negf_rgf_plan *plan, int on);
+int negf_rgf_plan_kernel_time(negf_rgf_plan *plan, double *total_ms,
+                              int *launches);
+int negf_rgf_plan_destroy(negf_rgf_plan *plan);
+
+/* Gauss-Legendre path (c_IntegrationPath::Define_GaussLegendrePoints
+ * IntegrationPath.cpp:41-147 over Quadrature::Gauss_Legendre
+ * CodeUtil.cpp:640-684): type 0 line, 1 circular arc.  Host arrays of n. */
+int negf_gl_path(negf_complex a, negf_complex b, int n, int type,
+                 negf_complex *E, double *w, negf_complex *mul);
+
 /* ------------------------------------------------------------------------
  * Level 2: NEGF material (one zigzag CNT in mode space).  Mirrors
  * c_NEGF_Common<T> + c_CNT (CNT.H:13-103) with the reference's input keys.
