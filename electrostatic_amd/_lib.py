@@ -97,7 +97,10 @@ SIGNATURES = {
     "negf_tube_positions": (C.c_int, [vp, dp]),
     "negf_tube_modes": (C.c_int, [vp, ip, dp, ip]),
     "negf_tube_set_potential": (C.c_int, [vp, dp, dp]),
+    "negf_tube_set_potential_device": (C.c_int, [vp, vp, vp]),
     "negf_tube_set_contacts": (C.c_int, [vp, dp, dp]),
+    "negf_tube_status": (C.c_int, [vp]),
+    "negf_tube_noneq_integrand": (C.c_int, [vp, C.c_int, dp, dp]),
     "negf_compute_rho0": (C.c_int, [vp, dp]),
     "negf_solve": (C.c_int, [vp, C.c_int, dp, ip]),
     "negf_solve_device": (C.c_int, [vp, C.c_int, vp, ip, vp]),
@@ -118,6 +121,7 @@ SIGNATURES = {
     "negf_scf_destroy": (C.c_int, [vp]),
     "negf_scf_bias_point": (C.c_int, [vp, C.c_double, dp, dp, ip, dp]),
     "negf_scf_kernel": (C.c_int, [vp, dp]),
+    "negf_scf_set_external_potential": (C.c_int, [vp, dp]),
     "negf_write_step_files": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp]),
 }
 
@@ -137,7 +141,7 @@ def load():
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name, None)
         if fn is None:
-            continue
+            raise RuntimeError(f"{LIB_PATH} does not export {name}: rebuild it")
         fn.restype = res
         fn.argtypes = args
     _lib = L

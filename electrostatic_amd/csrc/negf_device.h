@@ -42,7 +42,7 @@ __device__ __forceinline__ double2 csqrt(double2 z)
 // Closed-form surface Green's function of the semi-infinite two-site-periodic
 // lead (reference CNT.cpp:354-393): z = E-U, F = z^2+g^2-b^2,
 // S = sqrt(F^2-4 z^2 g^2), root (F+S)/(2 g^2 z) if Im<0 else (F-S)/(...).
-// Sets *ok = 0 when neither root has Im<0 (the smaller root is returned).
+// Sets *ok = 0 when neither root has Im<0 (and returns 0).
 __device__ __forceinline__ double2 surface_gf(double2 E, double U, double beta, double gamma, int* ok)
 {
     double2 z = make_double2(E.x - U, E.y);
@@ -56,8 +56,10 @@ __device__ __forceinline__ double2 surface_gf(double2 E, double U, double beta, 
     double2 v2 = cdiv(csub(F, S), D);
     if (v1.y < 0.0) { *ok = 1; return v1; }
     if (v2.y < 0.0) { *ok = 1; return v2; }
+    // neither root: g = 0, the zero-initialised block the reference keeps
+    // (CNT.cpp:376-381); flagged in the launch status word
     *ok = 0;
-    return cabs2(v1) <= cabs2(v2) ? v1 : v2;
+    return make_double2(0.0, 0.0);
 }
 
 // 1-D decimation (reference NEGF_Common.cpp:1436-1508 condensed Hamiltonian

@@ -176,6 +176,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         w.Uc0 = tb.Uc0;
         w.Uc1 = tb.Uc1;
         w.U = tb.dU;
+        w.Ucp = tb.dUc;
         w.chain0 = tb.chain0;
         w.nchain = tb.nchain;
         w.item0 = items;
@@ -373,9 +374,12 @@ struct L1Cache {
     DevBuf chains, pairs;
     RgfPlan plan;
 };
-static L1Cache& l1() {
-    static L1Cache c;
-    return c;
+// Never destroyed: device memory must not be freed from static destructors
+// that may run after the HIP runtime has shut down.
+static L1Cache& l1()
+{
+    static L1Cache* c = new L1Cache;
+    return *c;
 }
 
 static int check_params(const negf_chain_params* p, const void* dU, int nE, const void* dE)
@@ -422,13 +426,14 @@ static DecParams dec_of(const negf_chain_params* p)
     return d;
 }
 
-static int finish_status(RgfPlan& plan, hipStream_t st)
+// Synchronise; report rootless surface GFs (g = 0 used) only if asked.
+static int finish_status(RgfPlan& plan, hipStream_t st, bool report_root = false)
 {
     int status = 0;
     int rc = plan.check_status(st, &status);
     if (rc) return rc;
-    if (status & 1) {
-        set_error("surface Green's function: no root with Im<0 at some energy");
+    if ((status & 1) && report_root) {
+        set_error("surface Green's function: no root with Im<0 at some energy (g = 0 used)");
         return NEGF_EROOT;
     }
     return NEGF_OK;
@@ -579,7 +584,7 @@ int negf_rgf_plan_status(negf_rgf_plan* h, void* stream)
         set_error("plan is NULL");
         return NEGF_EARG;
     }
-    return finish_status(h->plan, (hipStream_t)stream);
+    return finish_status(h->plan, (hipStream_t)stream, true);
 }
 
 int negf_rgf_plan_enable_timing(negf_rgf_plan* h, int on)

@@ -24,8 +24,12 @@ void clear_error();
         }                                                                                   \
     } while (0)
 
-// Grow-only device buffer.
+// Grow-only device buffer (released by its owner or on destruction).
 struct DevBuf {
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
     void* p = nullptr;
     size_t cap = 0;
     int ensure(size_t bytes);
@@ -36,6 +40,10 @@ struct DevBuf {
 
 // Grow-only pinned host buffer.
 struct HostBuf {
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
     void* p = nullptr;
     size_t cap = 0;
     int ensure(size_t bytes);
@@ -53,6 +61,7 @@ struct RgfTube {
     int N = 0;
     const double* dU = nullptr;   // device potential, N values
     double Uc0 = 0, Uc1 = 0;      // contact potentials
+    const double* dUc = nullptr;  // or: device contact potentials (2 values)
     double t_odd = 0;             // gamma
     int sample[3] = {-1, -1, -1}; // integrand sample sites (KIND_NONEQ)
     int chain0 = 0, nchain = 0;   // chains [chain0, chain0+nchain) of d_chains

@@ -55,6 +55,7 @@ struct WTube {
     long long part_off;   // this tube's partial rows (groups x N)
     int sample[3];        // non-equilibrium integrand sample sites (-1: none)
     int pad;
+    const double* Ucp;    // device contact potentials (2 values), or null: Uc0/Uc1
 };
 
 struct WLaunch {
@@ -623,14 +624,16 @@ __global__ __launch_bounds__(256) void sigma_kernel(ChainDesc* chains, const int
     ChainDesc* ch = chains + c;
     const double2 E = make_double2(ch->Er, ch->Ei);
     const double te = ch->t_even, to = tube.t_odd;
+    const double Uc0 = tube.Ucp ? tube.Ucp[0] : tube.Uc0;
+    const double Uc1 = tube.Ucp ? tube.Ucp[1] : tube.Uc1;
     double2 gL, gR;
     int okL = 1, okR = 1;
     if (use_decimation) {
-        gL = decimation_gf(make_double2(E.x - tube.Uc0, E.y), te, to, dec_layers, dec_max_iter, dec_rel_error);
-        gR = decimation_gf(make_double2(E.x - tube.Uc1, E.y), te, to, dec_layers, dec_max_iter, dec_rel_error);
+        gL = decimation_gf(make_double2(E.x - Uc0, E.y), te, to, dec_layers, dec_max_iter, dec_rel_error);
+        gR = decimation_gf(make_double2(E.x - Uc1, E.y), te, to, dec_layers, dec_max_iter, dec_rel_error);
     } else {
-        gL = surface_gf(E, tube.Uc0, te, to, &okL);
-        gR = surface_gf(E, tube.Uc1, te, to, &okR);
+        gL = surface_gf(E, Uc0, te, to, &okL);
+        gR = surface_gf(E, Uc1, te, to, &okR);
     }
     if (!okL || !okR) atomicOr(status, 1);
     ch->SLr = to * to * gL.x;

@@ -1,0 +1,261 @@
+"""Python face of the C ABI for tests, benchmarks and scripting.
+
+Mirrors the reference's NEGF material interface (c_NEGF_Common / c_CNT,
+Source/Solver/Transport/NEGF/NEGF_Common.H:478-605) with its input key names;
+every call goes to libnegf_amd.so (HIP, gfx950).  No computation happens in
+Python beyond argument marshalling.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def tube_params(**keys):
+    """negf_tube_params with the reference defaults, overridden by input keys,
+    e.g. num_unitcells=100, eq_integration_pts=(30, 30, 64),
+    noneq_integration_pts=(200,), contact_mu=(-1, -1), E_zPlus_imag=1e-5.
+    noneq_percent_intercuts defaults to uniform cuts as in the reference."""
+    lib = _lib.load()
+    p = _lib.negf_tube_params()
+    lib.negf_tube_params_default(C.byref(p))
+    for k, v in keys.items():
+        if not hasattr(p, k):
+            raise KeyError(f"unknown tube parameter {k}")
+        cur = getattr(p, k)
+        if hasattr(cur, "__len__"):
+            vals = list(v)
+            if len(vals) > len(cur):
+                raise ValueError(f"{k}: at most {len(cur)} values")
+            for i, x in enumerate(vals):
+                cur[i] = x
+        else:
+            setattr(p, k, v)
+    if "noneq_integration_pts" in keys and "num_noneq_paths" not in keys:
+        p.num_noneq_paths = len(list(keys["noneq_integration_pts"]))
+    return p
+
+
+class Tube:
+    """One NEGF tube (c_CNT in mode space) on the GPU."""
+
+    def __init__(self, params=None, **keys):
+        self.lib = _lib.load()
+        self.params = params if params is not None else tube_params(**keys)
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_tube_create(C.byref(self.params), C.byref(h)))
+        self.h = h
+        self.N = self.lib.negf_tube_num_sites(h)
+
+    def close(self):
+        if self.h:
+            self.lib.negf_tube_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def positions(self):
+        out = np.zeros(self.N)
+        _lib.check(self.lib.negf_tube_positions(self.h, _dp(out)))
+        return out
+
+    def modes(self):
+        ids = np.zeros(32, np.int32)
+        beta = np.zeros(32)
+        deg = np.zeros(32, np.int32)
+        n = self.lib.negf_tube_modes(self.h, _ip(ids), _dp(beta), _ip(deg))
+        if n < 0:
+            _lib.check(n)
+        return ids[:n].copy(), beta[:n].copy(), deg[:n].copy()
+
+    def set_potential(self, U, Uc=None):
+        U = np.ascontiguousarray(U, np.float64)
+        Uc = np.ascontiguousarray(Uc if Uc is not None else (U[0], U[-1]), np.float64)
+        _lib.check(self.lib.negf_tube_set_potential(self.h, _dp(U), _dp(Uc)))
+
+    def set_contacts(self, mu, T=(298.0, 298.0)):
+        m = np.ascontiguousarray(mu, np.float64)
+        t = np.ascontiguousarray(T, np.float64)
+        _lib.check(self.lib.negf_tube_set_contacts(self.h, _dp(m), _dp(t)))
+
+    def rho0(self):
+        out = np.zeros(self.N)
+        _lib.check(self.lib.negf_compute_rho0(self.h, _dp(out)))
+        return out
+
+    def solve(self, iter=0):
+        """Solve_NEGF: induced charge per site (e) and integration points."""
+        Q = np.zeros(self.N)
+        n = C.c_int()
+        _lib.check(self.lib.negf_solve(self.h, iter, _dp(Q), C.byref(n)))
+        return Q, n.value
+
+    def solve_device(self, iter, dQ_ptr, stream=None):
+        n = C.c_int()
+        _lib.check(self.lib.negf_solve_device(self.h, iter, C.c_void_p(dQ_ptr), C.byref(n),
+                                              C.c_void_p(stream) if stream else None))
+        return n.value
+
+    def status(self):
+        _lib.check(self.lib.negf_tube_status(self.h))
+
+    def charge_components(self):
+        eq, neq, r0 = np.zeros(self.N), np.zeros(self.N), np.zeros(self.N)
+        _lib.check(self.lib.negf_get_charge_components(self.h, _dp(eq), _dp(neq), _dp(r0)))
+        return eq, neq, r0
+
+    def limits(self):
+        out = np.zeros(10)
+        _lib.check(self.lib.negf_tube_limits(self.h, _dp(out)))
+        keys = ["mu_min", "mu_max", "kT_min", "kT_max", "flag_noneq", "num_poles",
+                "E_contour_right", "E_zeta_imag", "E_eta", "E_rightmost"]
+        return dict(zip(keys, out))
+
+    def noneq_layout(self):
+        n = C.c_int()
+        pts = np.zeros(8, np.int32)
+        cuts = np.zeros(8)
+        _lib.check(self.lib.negf_tube_noneq_layout(self.h, C.byref(n), _ip(pts), _dp(cuts)))
+        return pts[:n.value].copy(), cuts[:max(0, n.value - 1)].copy()
+
+    def noneq_integrand(self):
+        n = self.lib.negf_tube_noneq_integrand(self.h, 0, None, None)
+        if n < 0:
+            _lib.check(-n)
+        E = np.zeros(max(n, 1))
+        I = np.zeros(3 * max(n, 1))
+        n = self.lib.negf_tube_noneq_integrand(self.h, n, _dp(E), _dp(I))
+        return E[:n], I[:3 * n].reshape(3, n)
+
+    def dos(self, E_real):
+        E = np.ascontiguousarray(E_real, np.float64)
+        d = np.zeros(E.size)
+        t = np.zeros(E.size)
+        _lib.check(self.lib.negf_tube_dos(self.h, E.size, _dp(E), _dp(d), _dp(t)))
+        return d, t
+
+
+class Broyden:
+    """Broyden modified-second mixer on the GPU (Broyden_Second_Parallel.cpp)."""
+
+    def __init__(self, S, fraction=0.1, max_step=100, norm_type=0):
+        self.lib = _lib.load()
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_broyden_create(S, fraction, max_step, norm_type, C.byref(h)))
+        self.h = h
+        self.S = S
+
+    def close(self):
+        if self.h:
+            self.lib.negf_broyden_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_nin(self, n):
+        n = np.ascontiguousarray(n, np.float64)
+        _lib.check(self.lib.negf_broyden_set_nin(self.h, _dp(n)))
+
+    def get_nin(self):
+        out = np.zeros(self.S)
+        _lib.check(self.lib.negf_broyden_get_nin(self.h, _dp(out)))
+        return out
+
+    def step(self, n_out):
+        n_out = np.ascontiguousarray(n_out, np.float64)
+        mx, l2 = C.c_double(), C.c_double()
+        per = np.zeros(self.S)
+        _lib.check(self.lib.negf_broyden_step(self.h, _dp(n_out), C.byref(mx), C.byref(l2), _dp(per)))
+        return mx.value, l2.value, per
+
+    def reset(self, keep_n_in=False, init_value=0.0):
+        _lib.check(self.lib.negf_broyden_reset(self.h, int(keep_n_in), init_value))
+
+    @property
+    def step_count(self):
+        return self.lib.negf_broyden_get_step(self.h)
+
+
+def scf_params(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0, Broyden_fraction=0.1,
+               Broyden_max_norm=1e-6, Broyden_norm_type=0, Broyden_threshold_maxstep=200,
+               reset_with_previous_charge_distribution=1, NS_initial_deposit_value=0.0):
+    p = _lib.negf_scf_params()
+    p.eps_r = eps_r
+    p.R_ring_nm = R_ring_nm
+    p.R_image_nm = R_image_nm
+    p.Broyden_fraction = Broyden_fraction
+    p.Broyden_max_norm = Broyden_max_norm
+    p.Broyden_norm_type = Broyden_norm_type
+    p.Broyden_threshold_maxstep = Broyden_threshold_maxstep
+    p.reset_with_previous_charge_distribution = reset_with_previous_charge_distribution
+    p.NS_initial_deposit_value = NS_initial_deposit_value
+    return p
+
+
+class Scf:
+    """Self-consistent loop of one tube (c_TransportSolver::Solve) with the
+    dense electrostatic stand-in."""
+
+    def __init__(self, tube, params=None, **keys):
+        self.lib = _lib.load()
+        self.tube = tube
+        self.params = params if params is not None else scf_params(**keys)
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_scf_create(tube.h, C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.negf_scf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def kernel(self):
+        N = self.tube.N
+        K = np.zeros(N * N)
+        _lib.check(self.lib.negf_scf_kernel(self.h, _dp(K)))
+        return K.reshape(N, N)
+
+    def set_external_potential(self, U_ext):
+        if U_ext is None:
+            _lib.check(self.lib.negf_scf_set_external_potential(self.h, None))
+            return
+        self._U_ext = np.ascontiguousarray(U_ext, np.float64)
+        _lib.check(self.lib.negf_scf_set_external_potential(self.h, _dp(self._U_ext)))
+
+    def bias_point(self, V_gate):
+        N = self.tube.N
+        U, Q = np.zeros(N), np.zeros(N)
+        it, nrm = C.c_int(), C.c_double()
+        _lib.check(self.lib.negf_scf_bias_point(self.h, V_gate, _dp(U), _dp(Q), C.byref(it), C.byref(nrm)))
+        return U, Q, it.value, nrm.value
+
+
+def write_step_files(prefix, ptd, Q=None, U=None, norm=None):
+    lib = _lib.load()
+    ptd = np.ascontiguousarray(ptd, np.float64)
+    arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (Q, U, norm)]
+    ptrs = [None if a is None else _dp(a) for a in arrs]
+    _lib.check(lib.negf_write_step_files(prefix.encode(), ptd.size, _dp(ptd), *ptrs))

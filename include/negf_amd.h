@@ -22,11 +22,14 @@ extern "C" {
 
 typedef struct { double re, im; } negf_complex;
 
-/* Status codes.  The reference aborts (amrex::Abort) where these are returned. */
+/* Status codes.  Where the reference aborts (amrex::Abort) these are returned. */
 #define NEGF_OK 0
 #define NEGF_EARG 1      /* invalid argument                                   */
 #define NEGF_EALLOC 2    /* device allocation failed                           */
-#define NEGF_EROOT 3     /* no surface-GF root with Im<0 (CNT.cpp:379-382)      */
+#define NEGF_EROOT 3     /* diagnostic only: some energy had no surface-GF root */
+                         /* with Im<0; g = 0 was used there, as the reference  */
+                         /* does (CNT.cpp:376-381).  Returned by the status    */
+                         /* calls, never by the compute calls.                 */
 #define NEGF_ENONCONV 4  /* Broyden_Step > Broyden_threshold_maxstep            */
                          /* (Transport.cpp:389-393)                             */
 #define NEGF_ENAN 5      /* non-finite value                                   */
@@ -95,7 +98,7 @@ int negf_rgf_plan_create(const negf_chain_params *p, const double *d_U,
                          negf_rgf_plan **out);
 int negf_rgf_plan_execute(negf_rgf_plan *plan, double *d_acc,
                           negf_complex *d_diagG, void *stream);
-/* Synchronise the stream and report NEGF_EROOT if any energy had no
+/* Synchronise the stream; NEGF_EROOT (diagnostic) if any energy had no
  * surface-GF root with Im<0 since the last check. */
 int negf_rgf_plan_status(negf_rgf_plan *plan, void *stream);
 /* hipEvent timing of the RGF kernel alone (on the execute stream). */
@@ -156,6 +159,9 @@ int negf_tube_modes(const negf_tube *t, int *mode_id, double *beta, int *degen);
 /* Potential at sites and contacts: U_n = -<phi>, U_contact = U at end sites
  * (Obtain_PotentialAtSites Nanostructure.cpp:494-504). Host arrays. */
 int negf_tube_set_potential(negf_tube *t, const double *U, const double Uc[2]);
+/* Device form used by a device-resident SCF loop: d_U (N doubles) is copied
+ * on the stream and the contact potentials become U[0], U[N-1]. */
+int negf_tube_set_potential_device(negf_tube *t, const double *d_U, void *stream);
 /* set_Contact_Electrochemical_Potential (NEGF_Common.cpp:356-368) and
  * contact temperatures; limits and paths are rebuilt at the next solve. */
 int negf_tube_set_contacts(negf_tube *t, const double mu[2], const double T[2]);
@@ -169,6 +175,9 @@ int negf_solve(negf_tube *t, int iter, double *Q, int *n_intg_pts);
 /* Same, with Q left in device memory d_Q (N doubles). */
 int negf_solve_device(negf_tube *t, int iter, double *d_Q, int *n_intg_pts,
                       void *stream);
+/* Synchronise the tube's work; NEGF_EROOT (diagnostic) if any energy of the
+ * solves since the last check had no surface-GF root with Im<0. */
+int negf_tube_status(negf_tube *t);
 /* Charge components of the last solve (_chargeComp.dat, :3000-3023):
  * Im RhoEq, Re RhoNonEq, Im Rho0 (host arrays of N). */
 int negf_get_charge_components(const negf_tube *t, double *eq_im,
@@ -181,6 +190,12 @@ int negf_tube_limits(const negf_tube *t, double *out);
  * per path points and percent intercuts. */
 int negf_tube_noneq_layout(const negf_tube *t, int *npaths, int *pts,
                            double *intercuts);
+/* Non-equilibrium integrand of the last adaptive update at sites N/2, N/4,
+ * 3N/4 (NonEq_Integrand, _Source, _Drain; NEGF_Common.cpp:3457-3495,
+ * _integrand.dat): E_real[n], integrand[3][n].  Returns n (points), or the
+ * required size if max_pts is smaller. */
+int negf_tube_noneq_integrand(const negf_tube *t, int max_pts, double *E_real,
+                              double *integrand);
 /* Compute_DensityOfStates over nE real energies (E_zPlus added):
  * DOS = spin * sum_modes degen * trace / (2 pi) / atoms_per_unitcell and
  * T = sum_modes degen * Gamma_L Gamma_R |G_0,N-1|^2 (host arrays). */
@@ -214,7 +229,7 @@ int negf_broyden_get_step(const negf_broyden *b);
 /* ------------------------------------------------------------------------
  * SCF driver (c_TransportSolver::Solve Transport.cpp:359-539) over one tube
  * with a dense 1/r-with-image electrostatic stand-in for the MLMG solve and
- * gather: U_i = -V_gate - sum_j K_ij n_in_j.
+ * gather: U_i = U_ext_i - V_gate - sum_j K_ij n_in_j.
  * ------------------------------------------------------------------------ */
 typedef struct {
     double eps_r, R_ring_nm, R_image_nm;   /* Coulomb stand-in            */
@@ -231,6 +246,10 @@ int negf_scf_destroy(negf_scf *s);
  * Reset_ForNextBiasStep.  U_out, Q_out: host arrays of N. */
 int negf_scf_bias_point(negf_scf *s, double V_gate, double *U_out,
                         double *Q_out, int *iters, double *final_norm);
+/* Fixed external potential U_ext (host array of N, eV) added to every
+ * iterate: U = U_ext - V_gate - K n_in.  NULL removes it.  Stands in for the
+ * fixed part of the electrostatic solution (work functions, fixed charges). */
+int negf_scf_set_external_potential(negf_scf *s, const double *U_ext);
 /* Dense kernel K (N x N, row-major, V per e) used by the stand-in. */
 int negf_scf_kernel(const negf_scf *s, double *K);
 

@@ -119,7 +119,9 @@ int surface_gf(C E, double U, double beta, double gamma, C *g)
         *g = val2;
         return 0;
     }
-    *g = abs2(val1) <= abs2(val2) ? val1 : val2;
+    /* neither root: the reference leaves gr at its zero initial value
+     * (MatrixBlock of GpuComplex, CNT.cpp:376-381); flagged for tests */
+    *g = mk(0., 0.);
     return 1;
 }
 
@@ -719,7 +721,10 @@ int orc_solve_negf(const orc_tube *t, const double *U, const double Uc[2],
 
 /* Find_NonEq_Percent_Intercuts_Adaptively NEGF_Common.cpp:2062-2185 for
  * num_noneq_paths == 3 or 2, driven by the Hsize/2 integrand maximum
- * (:3525-3555).  Updates t->noneq_intercuts and t->noneq_pts in place. */
+ * (:3525-3555).  The point densities per kT are computed at the first call
+ * unless given (flag_noneq_integration_pts_density) and kept; E_at_max keeps
+ * its previous value when every integrand sample is zero.  Updates
+ * t->noneq_intercuts, t->noneq_pts, t->noneq_density and t->E_at_max. */
 int orc_adaptive_intercuts(orc_tube *t, const double *U, const double Uc[2])
 {
     orc_limits L;
@@ -729,29 +734,32 @@ int orc_adaptive_intercuts(orc_tube *t, const double *U, const double Uc[2])
     int total = 0;
     for (auto &p : paths) total += int(p.E.size());
     std::vector<orc_c> neq(t->N);
-    std::vector<double> integ(3 * total);
+    std::vector<double> integ(3 * std::max(total, 1));
     int np = 0;
     int st = orc_rho_noneq(t, U, Uc, neq.data(), integ.data(), &np);
-    double maxv = 0., E_at_max = 0.;
+    double maxv = 0.;
     int e_glo = 0;
     for (auto &p : paths)
         for (size_t e = 0; e < p.E.size(); ++e, ++e_glo)
             if (maxv < std::fabs(integ[e_glo])) {
                 maxv = std::fabs(integ[e_glo]);
-                E_at_max = p.E[e].re;
+                t->E_at_max = p.E[e].re;
             }
     double kT0 = L.kT_contact[0];
-    double E_star = (E_at_max - t->mu[0]) / kT0;
+    double E_star = (t->E_at_max - t->mu[0]) / kT0;
     double E_min = (L.E_contour_right.re - t->mu[0]) / kT0;
     double E_max = (L.E_rightmost.re - t->mu[0]) / kT0;
+    double *d = t->noneq_density;
     if (t->num_noneq_paths == 3) {
-        double d[3];
-        d[0] = t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min));
-        d[1] = t->noneq_pts[1] /
-               (((t->noneq_intercuts[1] - t->noneq_intercuts[0]) / 100.) *
-                (E_max - E_min));
-        d[2] = t->noneq_pts[2] /
-               (((100. - t->noneq_intercuts[1]) / 100.) * (E_max - E_min));
+        if (!t->has_density) {
+            d[0] = t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min));
+            d[1] = t->noneq_pts[1] /
+                   (((t->noneq_intercuts[1] - t->noneq_intercuts[0]) / 100.) *
+                    (E_max - E_min));
+            d[2] = t->noneq_pts[2] /
+                   (((100. - t->noneq_intercuts[1]) / 100.) * (E_max - E_min));
+            t->has_density = 1;
+        }
         double E_lower = std::max(E_star - t->kT_window[0], E_min);
         double E_upper = std::min(E_star + t->kT_window[1], E_max);
         t->noneq_intercuts[0] = 100 * (E_lower - E_min) / (E_max - E_min);
@@ -760,10 +768,12 @@ int orc_adaptive_intercuts(orc_tube *t, const double *U, const double Uc[2])
         t->noneq_pts[1] = int(std::round(((E_upper - E_lower) * d[1]) / 2) * 2);
         t->noneq_pts[2] = int(std::round(((E_max - E_upper) * d[2]) / 2) * 2);
     } else if (t->num_noneq_paths == 2) {
-        double d[2];
-        d[0] = t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min));
-        d[1] = t->noneq_pts[1] /
-               (((100 - t->noneq_intercuts[0]) / 100.) * (E_max - E_min));
+        if (!t->has_density) {
+            d[0] = t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min));
+            d[1] = t->noneq_pts[1] /
+                   (((100 - t->noneq_intercuts[0]) / 100.) * (E_max - E_min));
+            t->has_density = 1;
+        }
         double E_upper = E_star + t->kT_window[1];
         t->noneq_intercuts[0] = 100 * (E_upper - E_min) / (E_max - E_min);
         t->noneq_pts[0] = int(std::round(((E_upper - E_min) * d[0]) / 2) * 2);
@@ -1084,7 +1094,7 @@ void orc_coulomb_kernel(int N, const double *ptd_nm, const orc_coulomb *c,
 }
 
 /* One bias point of c_TransportSolver::Solve Transport.cpp:359-539 with the
- * dense stand-in for Parts 1-2 (U = -V_gate - K n_in; U_contact = U at the
+ * dense stand-in for Parts 1-2 (U = U_ext - V_gate - K n_in; U_contact = U at the
  * end sites as in Nanostructure.cpp:494-504), Part 3 NEGF, Part 4 Broyden,
  * loop while Broyden_Norm > Broyden_max_norm, then Reset_ForNextBiasStep. */
 int orc_scf_bias_point(const orc_tube *t, const orc_c *Rho0, const double *K,
@@ -1106,7 +1116,7 @@ int orc_scf_bias_point(const orc_tube *t, const orc_c *Rho0, const double *K,
         for (int i = 0; i < N; ++i) {
             double s = 0.;
             for (int j = 0; j < N; ++j) s += K[size_t(i) * N + j] * nin[j];
-            U[i] = -V_gate - s;
+            U[i] = (sp->U_ext ? sp->U_ext[i] : 0.) - V_gate - s;
         }
         double Uc[2] = {U[0], U[N - 1]};
         st |= orc_solve_negf(t, U.data(), Uc, Rho0, Q.data(), nullptr, nullptr);

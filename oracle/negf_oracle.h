@@ -53,6 +53,9 @@ typedef struct {
     int decimation_rel_error;    /* int in the reference (NEGF_Common.H:475)   */
     /* adaptive integration limits (NEGF_Common.H:453-460) */
     double kT_window[2];
+    int has_density;             /* flag_noneq_integration_pts_density        */
+    double noneq_density[ORC_MAX_PATHS]; /* points per kT, fixed once set     */
+    double E_at_max;             /* E_at_max_noneq_integrand (persists)        */
 } orc_tube;
 
 typedef struct {
@@ -144,12 +147,17 @@ typedef struct {
     int max_step;        /* Broyden_threshold_maxstep                      */
     int reset_with_previous;
     double initial_value;
+    const double *U_ext; /* fixed external potential added to U (or NULL) */
 } orc_scf_params;
 
 int orc_scf_bias_point(const orc_tube *t, const orc_c *Rho0, const double *K,
                        double V_gate, const orc_scf_params *sp,
                        orc_broyden *b, double *U_out, double *Q_out,
                        int *iters, double *final_norm);
+
+/* ---- per-step output files (reference iostream formatting) ---- */
+void orc_write_step_files(const char *prefix, int N, const double *ptd,
+                          const double *Q, const double *U, const double *norm);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,9 @@ class orc_tube(C.Structure):
         ("decimation_max_iter", C.c_int),
         ("decimation_rel_error", C.c_int),
         ("kT_window", C.c_double * 2),
+        ("has_density", C.c_int),
+        ("noneq_density", C.c_double * MAX_PATHS),
+        ("E_at_max", C.c_double),
     ]
 
 
@@ -79,7 +82,8 @@ class orc_scf_params(C.Structure):
     _fields_ = [("fraction", C.c_double), ("max_norm", C.c_double),
                 ("norm_type", C.c_int), ("max_step", C.c_int),
                 ("reset_with_previous", C.c_int),
-                ("initial_value", C.c_double)]
+                ("initial_value", C.c_double),
+                ("U_ext", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -423,9 +427,11 @@ def coulomb_kernel(ptd_nm, eps_r=1.0, R_ring_nm=0.666, R_image_nm=3.0):
 
 def scf_bias_point(t, Rho0, K, V_gate, broyden, fraction=0.1, max_norm=1e-6,
                    norm_type=0, max_step=200, reset_with_previous=1,
-                   initial_value=0.0):
+                   initial_value=0.0, U_ext=None):
+    ext = None if U_ext is None else np.ascontiguousarray(U_ext, np.float64)
     sp = orc_scf_params(fraction, max_norm, norm_type, max_step,
-                        reset_with_previous, initial_value)
+                        reset_with_previous, initial_value,
+                        _dp(ext) if ext is not None else None)
     Rho0 = _carr(Rho0)
     K = np.ascontiguousarray(K, np.float64)
     U = np.zeros(t.N)
@@ -436,3 +442,10 @@ def scf_bias_point(t, Rho0, K, V_gate, broyden, fraction=0.1, max_norm=1e-6,
                                   C.byref(sp), broyden.h, _dp(U), _dp(Q),
                                   C.byref(it), C.byref(fn))
     return U, Q, it.value, fn.value, st
+
+
+def write_step_files(prefix, ptd, Q=None, U=None, norm=None):
+    ptd = np.ascontiguousarray(ptd, np.float64)
+    arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (Q, U, norm)]
+    lib().orc_write_step_files(prefix.encode(), ptd.size, _dp(ptd),
+                               *[None if a is None else _dp(a) for a in arrs])

@@ -1,0 +1,97 @@
+"""CPU checks of the C-ABI library (no GPU): it loads, exports every symbol
+include/negf_amd.h declares, and its host-only entry points (parameter
+defaults, quadrature paths, output writers) agree with the oracle."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from electrostatic_amd import _lib, negf
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "negf_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    names = re.findall(r"\b(negf_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if getattr(lib, n, None) is None]
+    assert not missing, missing
+    # and the ctypes table covers them all
+    assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
+
+
+def test_version_and_limits():
+    lib = _lib.load()
+    assert lib.negf_version() >= 100
+    assert lib.negf_max_sites() == 8192
+
+
+def test_tube_param_defaults_follow_the_reference():
+    p = negf.tube_params()
+    assert tuple(p.type_id) == (17, 0)
+    assert p.acc == 1.42e-10 and p.gamma == 2.5
+    assert p.E_valence_min == -14.0 and p.E_pole_max == 10.0 and p.E_zPlus_imag == 1e-8
+    assert tuple(p.Fermi_tail_factors) == (14.0, 14.0)
+    assert p.num_noneq_paths == 1
+    assert p.decimation_layers == 10 and p.decimation_max_iter == 30
+    assert p.decimation_rel_error == 0           # int(1.e-5) in the reference
+    assert p.integrand_correction_interval == 500
+    assert tuple(p.kT_window_around_singularity) == (1.0, 1.0)
+    assert tuple(p.contact_T) == (298.0, 298.0)
+    q = negf.tube_params(noneq_integration_pts=(50, 60, 70), contact_mu=(-1, -1.1))
+    assert q.num_noneq_paths == 3 and tuple(q.contact_mu) == (-1.0, -1.1)
+    with pytest.raises(KeyError):
+        negf.tube_params(no_such_key=1)
+
+
+@pytest.mark.parametrize("kind,a,b,n", [(0, -1 + 0.5j, 2 + 1j, 30), (1, -0.3 + 3.1j, -10 + 1e-5j, 64),
+                                        (0, -10 + 1e-8j, 0.4 + 1e-8j, 200)])
+def test_gl_path_matches_oracle(kind, a, b, n):
+    from electrostatic_amd.workloads import gl_path
+    E, w, mul = gl_path(a, b, n, kind)
+    Eo, wo, mo = O.gl_path(a, b, n, kind)
+    # same Newton iteration; compilers contract multiply-adds differently, and
+    # the outermost weights (taken at the previous Newton iterate) amplify it
+    assert (np.abs(w - wo) / wo).max() < 1e-11
+    assert np.abs(E - Eo).max() < 1e-14 * max(1.0, np.abs(Eo).max())
+    assert np.abs(mul - mo).max() < 1e-14 * max(1.0, np.abs(mo).max())
+
+
+def test_gl_path_rejects_bad_arguments():
+    lib = _lib.load()
+    z = _lib.negf_complex(0, 0)
+    rc = lib.negf_gl_path(z, z, 0, 0, None, None, None)
+    assert rc == 1 and b"invalid" in lib.negf_last_error()
+
+
+def test_step_files_match_reference_format(tmp_path):
+    rng = np.random.default_rng(0)
+    N = 37
+    ptd = np.sort(rng.normal(size=N)) * 7.3
+    Q = rng.normal(size=N) * 1e-3
+    Q[3] = 0.0
+    Q[5] = 1.25e-17
+    U = rng.normal(size=N)
+    nrm = np.abs(rng.normal(size=N)) * 1e-7
+    a = str(tmp_path / "prod")
+    b = str(tmp_path / "orc")
+    negf.write_step_files(a, ptd, Q=Q, U=U, norm=nrm)
+    O.write_step_files(b, ptd, Q=Q, U=U, norm=nrm)
+    for suffix in ("_Qout.dat", "_U.dat", "_norm.dat"):
+        pa = open(a + suffix, "rb").read()
+        pb = open(b + suffix, "rb").read()
+        assert pa == pb, suffix
+    lines = open(a + "_Qout.dat").read().splitlines()
+    assert lines[0] == "'axial location / (nm)', 'Induced charge per site / (e)'"
+    assert len(lines) == N + 1 and len(lines[1]) == 70

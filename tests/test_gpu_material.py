@@ -1,0 +1,246 @@
+"""GPU parity of the NEGF material path (negf_tube_*, negf_solve, Broyden,
+SCF) against the oracle on the same inputs.
+
+Tolerances: site charge 1e-10 relative (max |dQ| / max |Q|); converged
+potential 1e-6 V (max |dU|); Broyden iterates 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from electrostatic_amd import negf
+from electrostatic_amd.workloads import gaussian_bumps
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+Q_TOL = 1e-10
+U_TOL = 1e-6
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def pair(num_unitcells=100, num_modes=1, mu=(-1.0, -1.0), T=(298.0, 298.0),
+         eq_pts=(30, 30, 64), noneq_pts=(200,), intercuts=(), eta=1e-5,
+         E_valence_min=-10.0, E_pole_max=3.0, **extra):
+    """The same tube for the product and the oracle."""
+    keys = dict(num_unitcells=num_unitcells, num_modes=num_modes, contact_mu=mu,
+                contact_T=T, eq_integration_pts=eq_pts,
+                noneq_integration_pts=noneq_pts, E_zPlus_imag=eta,
+                E_valence_min=E_valence_min, E_pole_max=E_pole_max)
+    if intercuts:
+        keys["noneq_percent_intercuts"] = intercuts
+    keys.update(extra)
+    tube = negf.Tube(**keys)
+    ids, beta, deg = tube.modes()
+    ot = O.make_tube(tube.N, list(beta), list(deg), mu=mu, T=T, eq_pts=eq_pts,
+                     noneq_pts=noneq_pts, intercuts=intercuts, eta=eta,
+                     E_valence_min=E_valence_min, E_pole_max=E_pole_max,
+                     kT_window=extra.get("kT_window_around_singularity", (1.0, 1.0)),
+                     use_decimation=extra.get("use_decimation", 0),
+                     decimation_layers=extra.get("decimation_layers", 10),
+                     decimation_max_iter=extra.get("decimation_max_iter", 30),
+                     decimation_rel_error=extra.get("decimation_rel_error", 0))
+    return tube, ot
+
+
+def test_modes_and_positions_match_oracle():
+    tube, _ = pair(num_unitcells=10, num_modes=4)
+    ids, beta, deg = tube.modes()
+    mode, gap, odeg = O.cnt_sorted_modes(17, 0, 1.42e-10, 2.5)
+    assert list(ids) == list(mode[:4]) and list(deg) == list(odeg[:4])
+    assert np.allclose(beta, [O.cnt_beta(int(J), 17, 2.5) for J in mode[:4]], rtol=0, atol=1e-15)
+    assert np.allclose(tube.positions(), O.cnt_ring_positions(17, 10), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("modes", [1, 3])
+def test_rho0_matches_oracle(modes):
+    tube, ot = pair(num_unitcells=10, num_modes=modes)
+    r0, st = O.rho0(ot)
+    assert st == 0
+    assert rel(tube.rho0(), r0.imag) < Q_TOL
+
+
+@pytest.mark.parametrize("modes", [1, 2])
+def test_solve_equilibrium_matches_oracle(modes):
+    tube, ot = pair(num_modes=modes)
+    U = gaussian_bumps(tube.N, seed=11, amp=0.3)
+    Uc = (U[0], U[-1])
+    tube.set_potential(U, Uc)
+    Q, npts = tube.solve(1)
+    r0, _ = O.rho0(ot)
+    Qo, comps, npo, st = O.solve_negf(ot, U, Uc, r0)
+    assert st == 0 and npts == npo == 30 + 30 + 64
+    assert rel(Q, Qo) < Q_TOL
+    eq, neq, rho0 = tube.charge_components()
+    assert rel(eq, comps[0]) < Q_TOL
+    assert np.all(neq == 0.0)
+    assert rel(rho0, comps[2]) < Q_TOL
+    lim = tube.limits()
+    L = O.energy_limits(ot, (-1.0, -1.0))
+    assert lim["flag_noneq"] == 0 and lim["num_poles"] == L.num_poles
+
+
+@pytest.mark.parametrize("modes,N_uc", [(1, 100), (2, 37)])
+def test_solve_nonequilibrium_matches_oracle(modes, N_uc):
+    mu = (-1.0, -1.2)
+    tube, ot = pair(num_unitcells=N_uc, num_modes=modes, mu=mu)
+    U = gaussian_bumps(tube.N, seed=5, amp=0.3)
+    Uc = (U[0], U[-1])
+    tube.set_potential(U, Uc)
+    Q, npts = tube.solve(1)
+    r0, _ = O.rho0(ot)
+    Qo, comps, npo, st = O.solve_negf(ot, U, Uc, r0)
+    assert st == 0 and npts == npo == 200
+    assert rel(Q, Qo) < Q_TOL
+    eq, neq, _ = tube.charge_components()
+    assert rel(eq, comps[0]) < Q_TOL
+    assert rel(neq, comps[1]) < Q_TOL
+
+
+def test_contact_update_rebuilds_paths():
+    tube, ot = pair(num_unitcells=20)
+    U = gaussian_bumps(tube.N, seed=2, amp=0.2)
+    tube.set_potential(U)
+    tube.solve(1)
+    tube.set_contacts((-0.9, -1.05))
+    Q, npts = tube.solve(2)
+    ot.mu[0], ot.mu[1] = -0.9, -1.05
+    r0, _ = O.rho0(ot)
+    Qo, _, npo, _ = O.solve_negf(ot, U, (U[0], U[-1]), r0)
+    assert npts == npo and rel(Q, Qo) < Q_TOL
+
+
+@pytest.mark.parametrize("npaths", [2, 3])
+def test_adaptive_intercuts_match_oracle(npaths):
+    pts = (60, 80, 60)[:npaths]
+    cuts = (40.0, 60.0)[:npaths - 1]
+    mu = (-1.0, -1.15)
+    tube, ot = pair(num_unitcells=25, mu=mu, noneq_pts=pts, intercuts=cuts,
+                    flag_adaptive_integration_limits=1, integrand_correction_interval=2)
+    U = gaussian_bumps(tube.N, seed=9, amp=0.25)
+    Uc = (U[0], U[-1])
+    tube.set_potential(U, Uc)
+    # integrand on the input layout, as the reference computes it before the cut
+    _, integ_o, _ = O.rho_noneq(ot, U, Uc, with_integrand=True)
+    Q, npts = tube.solve(0)          # iter % interval == 0: correct the limits
+    E, integ = tube.noneq_integrand()
+    assert integ.shape == integ_o.shape
+    assert rel(integ, integ_o) < 1e-10
+    O.adaptive_intercuts(ot, U, Uc)
+    p_gpu, c_gpu = tube.noneq_layout()
+    assert list(p_gpu) == [ot.noneq_pts[i] for i in range(npaths)]
+    assert np.allclose(c_gpu, [ot.noneq_intercuts[i] for i in range(npaths - 1)], rtol=0, atol=1e-9)
+    r0, _ = O.rho0(ot)
+    Qo, _, npo, _ = O.solve_negf(ot, U, Uc, r0)
+    assert npts == npo and rel(Q, Qo) < Q_TOL
+    # iter 1: no correction, same layout; iter 2: corrected again with the
+    # densities of the first correction
+    Q1, _ = tube.solve(1)
+    assert rel(Q1, Qo) < Q_TOL
+    tube.solve(2)
+    O.adaptive_intercuts(ot, U, Uc)
+    p2, c2 = tube.noneq_layout()
+    assert list(p2) == [ot.noneq_pts[i] for i in range(npaths)]
+
+
+def test_decimation_leads_match_oracle():
+    tube, ot = pair(num_unitcells=20, mu=(-1.0, -1.1), use_decimation=1,
+                    decimation_layers=10, decimation_max_iter=30, decimation_rel_error=0)
+    U = gaussian_bumps(tube.N, seed=4, amp=0.2)
+    Uc = (U[0], U[-1])
+    tube.set_potential(U, Uc)
+    Q, _ = tube.solve(1)
+    r0, _ = O.rho0(ot)
+    Qo, _, _, _ = O.solve_negf(ot, U, Uc, r0)
+    assert rel(Q, Qo) < Q_TOL
+
+
+def test_dos_matches_oracle():
+    tube, ot = pair(num_unitcells=1, num_modes=6, eta=1e-8)
+    E = np.linspace(-2.0, 2.0, 97)
+    d, t = tube.dos(E)
+    do, to, st = O.dos_transmission(ot, np.zeros(tube.N), np.zeros(2), E + 1e-8j, 68)
+    assert rel(d, do) < 1e-10
+    assert np.abs(t - to).max() < 1e-10 * max(1.0, np.abs(to).max())
+
+
+def test_broyden_matches_oracle():
+    S = 300
+    rng = np.random.default_rng(3)
+    A = 0.3 * rng.normal(size=(S, S)) / np.sqrt(S)
+    b = rng.normal(size=S)
+    g = negf.Broyden(S, fraction=0.1, max_step=60, norm_type=0)
+    o = O.Broyden(S, 0.1, 60, 0)
+    n0 = rng.normal(size=S)
+    g.set_nin(n0)
+    o.set_nin(n0)
+    for k in range(25):
+        n_g = g.get_nin()
+        n_o = o.get_nin()
+        assert rel(n_g, n_o) < 1e-12, k
+        out = A @ n_o + b
+        mg, lg, pg = g.step(out)
+        mo, lo, po, st = o.step(out)
+        assert abs(mg - mo) <= 1e-12 * mo and abs(lg - lo) <= 1e-12 * lo
+        assert rel(pg, po) < 1e-12
+    assert g.step_count == o.step_count
+    g.reset(True)
+    o.reset(True)
+    assert rel(g.get_nin(), o.get_nin()) < 1e-12 and g.step_count == o.step_count == 1
+    g.reset(False, 0.25)
+    assert np.all(g.get_nin() == 0.25)
+
+
+@pytest.mark.parametrize("norm_type", [0, 1])
+def test_broyden_relative_norm(norm_type):
+    S = 64
+    rng = np.random.default_rng(8)
+    g = negf.Broyden(S, 0.2, 10, norm_type)
+    o = O.Broyden(S, 0.2, 10, norm_type)
+    n0 = 1.0 + rng.uniform(size=S)
+    g.set_nin(n0)
+    o.set_nin(n0)
+    for _ in range(5):
+        out = 0.5 * o.get_nin() + 1.0
+        mg, lg, _ = g.step(out)
+        mo, lo, _, _ = o.step(out)
+        assert abs(mg - mo) <= 1e-13 * mo
+    assert rel(g.get_nin(), o.get_nin()) < 1e-12
+
+
+def scf_pair(N_uc=100, mu=(-1.0, -1.0)):
+    tube, ot = pair(num_unitcells=N_uc, mu=mu)
+    sp = dict(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+    scf = negf.Scf(tube, Broyden_fraction=0.1, Broyden_max_norm=1e-6,
+                   Broyden_threshold_maxstep=200, **sp)
+    K = O.coulomb_kernel(tube.positions(), **sp)
+    assert rel(scf.kernel(), K) < 1e-14
+    return tube, ot, scf, K
+
+
+def test_scf_gate_sweep_matches_oracle():
+    """Metal-voltage sweep 0..1 V in 0.1 V steps, warm start between bias
+    points (reset_with_previous_charge_distribution = 1), 1 mode, N = 400."""
+    tube, ot, scf, K = scf_pair()
+    U_ext = gaussian_bumps(tube.N, seed=1, amp=0.3)
+    scf.set_external_potential(U_ext)
+    r0, _ = O.rho0(ot)
+    ob = O.Broyden(tube.N, 0.1, 200, 0)
+    for V in np.round(np.arange(0.0, 1.0 + 1e-9, 0.1), 10):
+        U, Q, it, nrm = scf.bias_point(V)
+        Uo, Qo, ito, nrmo, st = O.scf_bias_point(ot, r0, K, V, ob, U_ext=U_ext)
+        assert st == 0 and nrm <= 1e-6 and nrmo <= 1e-6
+        assert np.abs(U - Uo).max() < U_TOL, (V, np.abs(U - Uo).max())
+        assert abs(it - ito) <= 1, (V, it, ito)
+
+
+def test_scf_nonequilibrium_bias_point():
+    tube, ot, scf, K = scf_pair(N_uc=50, mu=(-1.0, -1.1))
+    r0, _ = O.rho0(ot)
+    ob = O.Broyden(tube.N, 0.1, 200, 0)
+    U, Q, it, nrm = scf.bias_point(0.2)
+    Uo, Qo, ito, nrmo, st = O.scf_bias_point(ot, r0, K, 0.2, ob)
+    assert st == 0
+    assert np.abs(U - Uo).max() < U_TOL
