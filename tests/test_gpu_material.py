@@ -183,8 +183,10 @@ def test_broyden_matches_oracle():
         out = A @ n_o + b
         mg, lg, pg = g.step(out)
         mo, lo, po, st = o.step(out)
-        assert abs(mg - mo) <= 1e-12 * mo and abs(lg - lo) <= 1e-12 * lo
-        assert rel(pg, po) < 1e-12
+        # norms are differences n_in - n_out of O(1) values: compare on that scale
+        scale = 1e-13 * np.abs(out).max()
+        assert abs(mg - mo) <= scale and abs(lg - lo) <= scale * np.sqrt(S)
+        assert np.abs(pg - po).max() <= scale
     assert g.step_count == o.step_count
     g.reset(True)
     o.reset(True)
@@ -202,11 +204,14 @@ def test_broyden_relative_norm(norm_type):
     n0 = 1.0 + rng.uniform(size=S)
     g.set_nin(n0)
     o.set_nin(n0)
-    for _ in range(5):
-        out = 0.5 * o.get_nin() + 1.0
-        mg, lg, _ = g.step(out)
-        mo, lo, _, _ = o.step(out)
-        assert abs(mg - mo) <= 1e-13 * mo
+    A = rng.normal(size=(S, S)) / np.sqrt(S)
+    for _ in range(8):
+        n = o.get_nin()
+        out = np.tanh(A @ n) + 1.5        # nonlinear: no exact secant convergence
+        mg, lg, pg = g.step(out)
+        mo, lo, po, _ = o.step(out)
+        assert abs(mg - mo) <= 1e-13 and abs(lg - lo) <= 1e-12
+        assert np.abs(pg - po).max() <= 1e-13
     assert rel(g.get_nin(), o.get_nin()) < 1e-12
 
 
