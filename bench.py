@@ -55,12 +55,69 @@ def cpu_baseline(wl, max_seconds=30.0):
             "sample": sample}
 
 
+def scf_sweep(with_cpu=True):
+    """SCF gate sweep of one (17,0) tube, N = 400 sites (100 unit cells), 1 mode,
+    contour 30/30/64 + 19 poles, eta = 1e-5 eV, contacts at -1 eV, external
+    potential = 4 seeded Gaussians (amplitude <= 0.3 eV), metal voltage 0..1 V
+    in 0.1 V steps, Broyden fraction 0.1, absolute norm <= 1e-6, warm start.
+    Reports bias points per second on the GPU (whole SCF loop per bias point,
+    device-resident) and for the oracle's serial SCF on one host thread."""
+    from electrostatic_amd import negf
+    from electrostatic_amd.workloads import gaussian_bumps
+    keys = dict(num_unitcells=100, num_modes=1, contact_mu=(-1.0, -1.0),
+                eq_integration_pts=(30, 30, 64), noneq_integration_pts=(200,),
+                E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0)
+    sp = dict(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+    tube = negf.Tube(**keys)
+    scf = negf.Scf(tube, Broyden_fraction=0.1, Broyden_max_norm=1e-6,
+                   Broyden_threshold_maxstep=200, **sp)
+    U_ext = gaussian_bumps(tube.N, seed=1, amp=0.3)
+    scf.set_external_potential(U_ext)
+    Vs = [round(0.1 * i, 10) for i in range(11)]
+    scf.bias_point(0.0)                      # warm-up (plans, first solve)
+    scf.close()
+    scf = negf.Scf(tube, Broyden_fraction=0.1, Broyden_max_norm=1e-6,
+                   Broyden_threshold_maxstep=200, **sp)
+    scf.set_external_potential(U_ext)
+    iters = []
+    t0 = time.perf_counter()
+    for V in Vs:
+        U, Q, it, nrm = scf.bias_point(V)
+        iters.append(it)
+    dt = time.perf_counter() - t0
+    out = {"workload": "SCF gate sweep 0..1 V (11 bias points), (17,0) tube N=400, "
+                       "contour 30/30/64 + poles, Broyden abs norm 1e-6",
+           "bias_points_per_s": len(Vs) / dt, "ms_per_bias_point": 1e3 * dt / len(Vs),
+           "iterations": iters, "kind": "plain"}
+    if with_cpu:
+        from oracle import oracle as O
+        _, betas, degs = tube.modes()
+        ot = O.make_tube(tube.N, list(betas), list(degs), mu=(-1.0, -1.0),
+                         eq_pts=(30, 30, 64), noneq_pts=(200,), eta=1e-5,
+                         E_valence_min=-10.0, E_pole_max=3.0)
+        r0, _ = O.rho0(ot)
+        K = O.coulomb_kernel(tube.positions(), **sp)
+        ob = O.Broyden(tube.N, 0.1, 200, 0)
+        t0 = time.perf_counter()
+        for V in Vs:
+            Uo, Qo, ito, nrmo, st = O.scf_bias_point(ot, r0, K, V, ob, U_ext=U_ext)
+        dtc = time.perf_counter() - t0
+        out["cpu_bias_points_per_s"] = len(Vs) / dtc
+        out["cpu_kind"] = "port (oracle serial SCF, reference loop order), 1 thread"
+        out["max_abs_dU_last_bias_V"] = float(np.abs(U - Uo).max())
+    scf.close()
+    tube.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scf", action="store_true",
+                    help="skip the secondary SCF gate-sweep measurement")
     ap.add_argument("--write-diag-g", action="store_true",
                     help="also materialise diag G^R ([energy][site], 512 MiB)")
     args = ap.parse_args()
@@ -177,6 +234,8 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl)
     lib.negf_rgf_plan_destroy(plan)
+    if not args.no_scf:
+        out["scf"] = scf_sweep(with_cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "negf_tube_limits": (C.c_int, [vp, dp]),
     "negf_tube_noneq_layout": (C.c_int, [vp, ip, ip, dp]),
     "negf_tube_dos": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+    "negf_tube_current": (C.c_int, [vp, dp]),
     "negf_broyden_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]),
     "negf_broyden_destroy": (C.c_int, [vp]),
     "negf_broyden_set_nin": (C.c_int, [vp, dp]),
@@ -123,6 +124,13 @@ SIGNATURES = {
     "negf_scf_kernel": (C.c_int, [vp, dp]),
     "negf_scf_set_external_potential": (C.c_int, [vp, dp]),
     "negf_write_step_files": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp]),
+    "negf_write_qin": (C.c_int, [C.c_char_p, C.c_int, dp, dp]),
+    "negf_write_charge_components": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp]),
+    "negf_write_current_header": (C.c_int, [C.c_char_p]),
+    "negf_write_current_line": (C.c_int, [C.c_char_p, C.c_int, C.c_double, C.c_double, dp,
+                                          C.c_double, C.c_int, C.c_double]),
+    "negf_read_charge_distribution": (C.c_int, [C.c_char_p, C.c_int, dp]),
+    "negf_conductance": (C.c_int, [C.c_int, dp, dp, C.c_double, C.c_double, dp, dp]),
 }
 
 _lib = None

@@ -91,10 +91,25 @@ static void launch_sweep(int nblk, const WLaunch& lp, hipStream_t st)
     hipLaunchKernelGGL((sweep_kernel<NEGF_SUB_K, NEGF_SUB_NS, KIND, WG>), dim3(nblk), dim3(256), 0, st, lp);
 }
 
+static void launch_sweep_eq_full(int ns, int nblk, const WLaunch& lp, hipStream_t st)
+{
+    constexpr int K = NEGF_SUB_K;
+    switch (ns) {
+        case 1: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 1>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 2: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 2>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 3: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 3>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 4: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 4>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 5: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 5>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 6: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 6>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 7: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 7>), dim3(nblk), dim3(256), 0, st, lp); break;
+        default: hipLaunchKernelGGL((sweep_eq_full_kernel<K, 8>), dim3(nblk), dim3(256), 0, st, lp); break;
+    }
+}
+
 // ---------------------------------------------------------------- plan
 RgfPlan::~RgfPlan()
 {
-    DevBuf* bufs[] = {&d_tubes_,       &d_item_tube_,   &d_chain_tube_,   &d_sblk_tube_,    &d_sblk_chain_,
+    DevBuf* bufs[] = {&d_item_list_,   &d_tubes_,       &d_item_tube_,   &d_chain_tube_,   &d_sblk_tube_,    &d_sblk_chain_,
                       &d_Tm_,          &d_St_,          &d_Dd_,           &d_part_,         &d_tmp_,
                       &d_dos_part_,    &d_status_,      &d_red_src_off_,  &d_red_nrows_,    &d_red_N_,
                       &d_red_dst_off_, &d_red2_src_off_, &d_red2_nrows_,  &d_red2_dst_off_};
@@ -147,6 +162,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     ntubes_ = (int)tubes.size();
     std::vector<WTube> wt(ntubes_);
     std::vector<int> item_tube, chain_tube(nchains, 0), sblk_tube, sblk_chain;
+    std::vector<int> fast_items[NEGF_SUB_NS + 1], slow_items;
     std::vector<long long> rsrc, rdst, tdst, t2src;
     std::vector<int> rnrows, rN, t2rows;
     long long poff = 0, toff = 0;
@@ -183,13 +199,22 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         w.part_off = poff;
         for (int q = 0; q < 3; ++q) w.sample[q] = tb.sample[q];
         const int G = (tb.nchain + 63) / 64;
-        for (int i = 0; i < G * P; ++i) item_tube.push_back(t);
+        for (int i = 0; i < G * P; ++i) {
+            item_tube.push_back(t);
+            // full segments of the equilibrium charge go to the unrolled sweep
+            const int sgi = i % P;
+            const bool full = sgi * L + L <= tb.N;
+            if (kind == KIND_EQ && !write_G && full)
+                fast_items[L / NEGF_SUB_K].push_back(items + i);
+            else
+                slow_items.push_back(items + i);
+        }
         items += G * P;
         for (int c = 0; c < tb.nchain; ++c) chain_tube[tb.chain0 + c] = t;
-        const int cpb = 256 / P;  // chains per scan block (padding lanes of the last group included)
-        for (int c0 = 0; c0 < G * 64; c0 += cpb) {
+        w.group0 = (int)sblk_tube.size();
+        for (int g = 0; g < G; ++g) {  // one scan wavefront per chain group
             sblk_tube.push_back(t);
-            sblk_chain.push_back(c0);
+            sblk_chain.push_back(g);
         }
         rsrc.push_back(poff);
         rnrows.push_back(G);
@@ -216,6 +241,18 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     if (kind == KIND_DOS && (rc = d_dos_part_.ensure((size_t)items * 64 * sizeof(double)))) return rc;
     if ((rc = upload(d_tubes_, wt))) return rc;
     if ((rc = upload(d_item_tube_, item_tube))) return rc;
+    {
+        std::vector<int> all;
+        for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
+            fast_off_[ns] = (int)all.size();
+            fast_n_[ns] = (int)fast_items[ns].size();
+            all.insert(all.end(), fast_items[ns].begin(), fast_items[ns].end());
+        }
+        slow_off_ = (int)all.size();
+        slow_n_ = (int)slow_items.size();
+        all.insert(all.end(), slow_items.begin(), slow_items.end());
+        if ((rc = upload(d_item_list_, all))) return rc;
+    }
     if ((rc = upload(d_chain_tube_, chain_tube))) return rc;
     if ((rc = upload(d_sblk_tube_, sblk_tube))) return rc;
     if ((rc = upload(d_sblk_chain_, sblk_chain))) return rc;
@@ -283,16 +320,28 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     }
     const int nblk = (nitems_ + 3) / 4;
     hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3(nblk), dim3(256), 0, st, lp);
-    hipLaunchKernelGGL(scan_kernel, dim3(nsblk_), dim3(256), 0, st, lp);
-    if (kind_ == KIND_EQ) {
+    hipLaunchKernelGGL(scan_kernel, dim3((nsblk_ + 3) / 4), dim3(256), 0, st, lp, nsblk_);
+    for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
+        if (fast_n_[ns] == 0) continue;
+        WLaunch lf = lp;
+        lf.item_list = d_item_list_.as<int>() + fast_off_[ns];
+        lf.nlist = fast_n_[ns];
+        launch_sweep_eq_full(ns, (fast_n_[ns] + 3) / 4, lf, st);
+    }
+    lp.item_list = d_item_list_.as<int>() + slow_off_;
+    lp.nlist = slow_n_;
+    const int nblk_slow = (slow_n_ + 3) / 4;
+    if (slow_n_ == 0) {
+        // nothing left for the general sweep
+    } else if (kind_ == KIND_EQ) {
         if (write_G_)
-            launch_sweep<KIND_EQ, true>(nblk, lp, st);
+            launch_sweep<KIND_EQ, true>(nblk_slow, lp, st);
         else
-            launch_sweep<KIND_EQ, false>(nblk, lp, st);
+            launch_sweep<KIND_EQ, false>(nblk_slow, lp, st);
     } else if (kind_ == KIND_NONEQ) {
-        launch_sweep<KIND_NONEQ, false>(nblk, lp, st);
+        launch_sweep<KIND_NONEQ, false>(nblk_slow, lp, st);
     } else {
-        launch_sweep<KIND_DOS, false>(nblk, lp, st);
+        launch_sweep<KIND_DOS, false>(nblk_slow, lp, st);
     }
     NEGF_HIP_TRY(hipGetLastError());
     if (timing_) {

@@ -97,7 +97,9 @@ class RgfPlan {
     ChainDesc* d_chains_ = nullptr;
     DecParams dec_;
     std::vector<RgfTube> tubes_;
-    DevBuf d_tubes_, d_item_tube_, d_chain_tube_, d_sblk_tube_, d_sblk_chain_;
+    DevBuf d_tubes_, d_item_tube_, d_chain_tube_, d_sblk_tube_, d_sblk_chain_, d_item_list_;
+    int fast_off_[NEGF_SUB_NS + 1] = {}, fast_n_[NEGF_SUB_NS + 1] = {};  // unrolled sweep items by sub-blocks
+    int slow_off_ = 0, slow_n_ = 0;                                      // general sweep items
     DevBuf d_Tm_, d_St_, d_Dd_, d_part_, d_tmp_, d_dos_part_, d_status_;
     DevBuf d_red_src_off_, d_red_nrows_, d_red_N_, d_red_dst_off_;   // pass 1
     DevBuf d_red2_src_off_, d_red2_nrows_, d_red2_dst_off_;          // pass 2

@@ -42,6 +42,8 @@ class Tube {
     int take_root_flag();
     int total_integration_pts() const;
     int dos(int nE, const double* E_real, double* dos_out, double* trans_out);
+    // Terminal currents (A) over the current non-equilibrium path.
+    int current(double I[2]);
 
     negf_tube_params p{};
     int N = 0;

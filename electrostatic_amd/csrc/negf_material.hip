@@ -673,6 +673,46 @@ int Tube::dos(int nE, const double* E_real, double* dos_out, double* trans_out)
     return NEGF_OK;
 }
 
+// Compute_Current (NEGF_Common.cpp:4321-4760) on the non-equilibrium path.
+// At a contact site the reference's IF_k = Gamma_k f_k A_kk - Gamma_k Gn_kk
+// reduces to Gamma_k A^j_kk (f_k - f_j), j the other contact, and
+// Gamma_L A^R_00 = Gamma_R A^L_{N-1,N-1} = T(E) = Gamma_L Gamma_R |G_{0,N-1}|^2:
+//   I_0 = s q/h sum_E g w T(E) Re((f_L - f_R) mul),  I_1 = -I_0 (same sum with
+// f_R - f_L).  T(E) per chain comes from the DOS launch over the same chains.
+int Tube::current(double I[2])
+{
+    I[0] = I[1] = 0.0;
+    if (!L.noneq || n_neq == 0) return NEGF_OK;
+    int rc;
+    if ((rc = dos_pairs.ensure(sizeof(double) * 2 * (size_t)n_neq))) return rc;
+    RgfPlan plan;
+    std::vector<RgfTube> tubes{rgf_tube(d_U.as<double>(), d_Uc.as<double>(), n_neq)};
+    if ((rc = plan.build(KIND_DOS, false, tubes, neq_chains.as<ChainDesc>(), n_neq, dec()))) return rc;
+    if ((rc = plan.enqueue(nullptr, nullptr, dos_pairs.as<double>(), stream))) return rc;
+    std::vector<double> h(2 * (size_t)n_neq);
+    NEGF_HIP_TRY(hipMemcpyAsync(h.data(), dos_pairs.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+    int status = 0;
+    if ((rc = plan.check_status(stream, &status))) return rc;
+    root_flag |= status & 1;
+    const double q_e = 1.602176634e-19, h_eVperHz = 4.135667696e-15;   // PhysConst
+    const double cm = spin_degen * q_e / h_eVperHz;
+    const int nm = (int)beta.size();
+    double s0 = 0.0, s1 = 0.0;
+    for (int m = 0; m < nm; ++m)
+        for (int e = 0; e < n_neq_E; ++e) {
+            const cplx E = neq_path.E[e];
+            const cplx fL = fermi(E - mu[0], L.kT_contact[0]);
+            const cplx fR = fermi(E - mu[1], L.kT_contact[1]);
+            const double Tm = h[2 * ((size_t)m * n_neq_E + e) + 1];
+            const double base = cm * degen[m] * neq_path.w[e] * Tm;
+            s0 += base * ((fL - fR) * neq_path.mul[e]).real();
+            s1 += base * ((fR - fL) * neq_path.mul[e]).real();
+        }
+    I[0] = s0;
+    I[1] = s1;
+    return NEGF_OK;
+}
+
 }  // namespace negf
 
 // ======================================================================== C ABI
@@ -928,6 +968,16 @@ int negf_tube_noneq_integrand(const negf_tube* t, int max_pts, double* E_real, d
             for (int w = 0; w < 3; ++w) integrand[(size_t)w * n + e] = tb.integrand[(size_t)w * n + e];
     }
     return n;
+}
+
+int negf_tube_current(negf_tube* t, double* I)
+{
+    clear_error();
+    if (!t || !I) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    return t->t.current(I);
 }
 
 int negf_tube_dos(negf_tube* t, int nE, const double* E_real, double* dos, double* trans)

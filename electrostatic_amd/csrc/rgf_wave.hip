@@ -70,10 +70,12 @@ struct WLaunch {
     double2* diagG;           // optional [chain][N]
     double* chain_out;        // per-chain outputs
     double* dos_part;         // [item][64] per-segment DOS traces
-    const int* sblk_tube;     // scan blocks: tube
-    const int* sblk_chain;    // scan blocks: first local chain
+    const int* sblk_tube;     // chain groups: tube
+    const int* sblk_chain;    // chain groups: group index within the tube
     int* status;
     int kind;                 // ChainKind of the launch
+    const int* item_list;     // sweep launches: items of this launch (or null: all)
+    int nlist;
 };
 
 __device__ __forceinline__ int exp_of(double m) { return m == 0.0 ? 0 : __builtin_amdgcn_frexp_exp(m); }
@@ -191,8 +193,15 @@ struct ItemCtx {
 __device__ __forceinline__ ItemCtx item_ctx(const WLaunch& lp, int item, int lane, WTube& tube)
 {
     ItemCtx c;
-    c.t = lp.item_tube[item];
+    c.t = __builtin_amdgcn_readfirstlane(lp.item_tube[item]);
     tube = lp.tubes[c.t];
+    // the tube is the same for the whole wavefront: keep its integers scalar
+    tube.N = __builtin_amdgcn_readfirstlane(tube.N);
+    tube.L = __builtin_amdgcn_readfirstlane(tube.L);
+    tube.P = __builtin_amdgcn_readfirstlane(tube.P);
+    tube.item0 = __builtin_amdgcn_readfirstlane(tube.item0);
+    tube.chain0 = __builtin_amdgcn_readfirstlane(tube.chain0);
+    tube.nchain = __builtin_amdgcn_readfirstlane(tube.nchain);
     int local = item - tube.item0;
     c.g = local / tube.P;
     c.s = local - c.g * tube.P;
@@ -203,11 +212,22 @@ __device__ __forceinline__ ItemCtx item_ctx(const WLaunch& lp, int item, int lan
 }
 
 // ------------------------------------------------------------ transfer
+// The wave's segment of U is staged once in LDS with coalesced loads; the
+// per-site reads are then wave-uniform LDS broadcasts instead of a vector
+// load (and its wait) per site.
+__device__ __forceinline__ void stage_segment(double* us, const double* __restrict__ U, int lo, int hi, int lane)
+{
+    for (int i = lane; i < hi - lo; i += 64) us[i] = U[lo + i];
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void transfer_kernel(WLaunch lp)
 {
+    __shared__ double us_all[4][NEGF_SEG_MAX];
     const int lane = threadIdx.x & 63;
-    const int item = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int wv = threadIdx.x >> 6;
+    const int item = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
     if (item >= lp.nitems) return;
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
@@ -215,24 +235,25 @@ __global__ __launch_bounds__(256) void transfer_kernel(WLaunch lp)
     const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int lo = min(tube.N, ic.s * tube.L);
     const int hi = min(tube.N, lo + tube.L);
-    const double* __restrict__ U = tube.U;
+    double* us = us_all[wv];
+    stage_segment(us, tube.U, lo, hi, lane);
     double2 m00 = make_double2(1.0, 0.0), m01 = make_double2(0.0, 0.0);
     double2 m10 = make_double2(0.0, 0.0), m11 = make_double2(1.0, 0.0);
     int eT = 0;
-    for (int p0 = lo; p0 < hi; p0 += K) {
-        const int len = min(K, hi - p0);
+    for (int q0 = 0; q0 < hi - lo; q0 += K) {
+        const int len = min(K, hi - lo - q0);
         if (len == K) {
 #pragma unroll
             for (int i = 0; i < K; i += 2) {
-                mat_step<0>(k, U[p0 + i], m00, m01, m10, m11);
-                mat_step<1>(k, U[p0 + i + 1], m00, m01, m10, m11);
+                mat_step<0>(k, us[q0 + i], m00, m01, m10, m11);
+                mat_step<1>(k, us[q0 + i + 1], m00, m01, m10, m11);
             }
         } else {
             for (int i = 0; i < len; ++i) {
                 if (i & 1)
-                    mat_step<1>(k, U[p0 + i], m00, m01, m10, m11);
+                    mat_step<1>(k, us[q0 + i], m00, m01, m10, m11);
                 else
-                    mat_step<0>(k, U[p0 + i], m00, m01, m10, m11);
+                    mat_step<0>(k, us[q0 + i], m00, m01, m10, m11);
             }
         }
         renorm_mat(m00, m01, m10, m11, eT);
@@ -250,126 +271,91 @@ __global__ __launch_bounds__(256) void transfer_kernel(WLaunch lp)
 }
 
 // ------------------------------------------------------------ scan
-// One thread per (chain, segment); a block holds 256/P chains of one tube.
-__global__ __launch_bounds__(256) void scan_kernel(WLaunch lp)
+// One wavefront per 64-chain group, lane = chain: the segment states are
+// propagated through the P segment transfer matrices in order, so every read
+// of Tm and write of St is a coalesced 512-byte row.
+//   u: U_0 = (u_0, u_{-1}) = (1, Sigma_L/t_{-1});  U_hi = T_s U_lo
+//   v: the same recurrence read backwards, V_lo = T_s^{-1} V_hi with
+//      T_s^{-1} = adj(T_s) / det(T_s) and det(T_s) = t_{lo-1}/t_{hi-1}
+//      (product of the per-site determinants c_n);
+//      V_N = (v_N, v_{N-1}) = (Sigma_R/t_{N-1}, 1)
+//   D = t_{N-1} u_N - Sigma_R u_{N-1}.
+__global__ __launch_bounds__(256) void scan_kernel(WLaunch lp, int ngroups)
 {
-    const int t = lp.sblk_tube[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+    if (gi >= ngroups) return;
+    const int t = __builtin_amdgcn_readfirstlane(lp.sblk_tube[gi]);
+    const int g = __builtin_amdgcn_readfirstlane(lp.sblk_chain[gi]);
     const WTube tube = lp.tubes[t];
-    const int P = tube.P;
-    const int s = threadIdx.x & (P - 1);
-    const int lc = lp.sblk_chain[blockIdx.x] + threadIdx.x / P;  // local chain
-    if (lc >= ((tube.nchain + 63) & ~63)) return;  // whole chain beyond the last group
+    const int P = __builtin_amdgcn_readfirstlane(tube.P);
+    const int L = __builtin_amdgcn_readfirstlane(tube.L);
+    const int N = __builtin_amdgcn_readfirstlane(tube.N);
+    const int lc = g * 64 + lane;
     const bool valid = lc < tube.nchain;
-    const int lcc = valid ? lc : tube.nchain - 1;   // descriptor of padding lanes
-    const int g = lc >> 6, lane = lc & 63;          // slot of this (possibly padding) lane
-    const int item = tube.item0 + g * P + s;
-    const int chain = tube.chain0 + lcc;
+    const int chain = tube.chain0 + (valid ? lc : tube.nchain - 1);
     const ChainDesc ch = lp.chains[chain];
     const double te = ch.t_even, to = tube.t_odd;
-    const int N = tube.N;
     const double tau = ((N - 1) & 1) ? to : te;  // t_{N-1}
     const double2 SL = make_double2(ch.SLr, ch.SLi);
     const double2 SR = make_double2(ch.SRr, ch.SRi);
-    const int lo = min(N, s * tube.L);
+    const long long item0 = tube.item0 + (long long)g * P;
 
-    const double* T = lp.Tm + (long long)item * 9 * 64 + lane;
-    double2 t00 = make_double2(T[0], T[64]), t01 = make_double2(T[128], T[192]);
-    double2 t10 = make_double2(T[256], T[320]), t11 = make_double2(T[384], T[448]);
-    const int tE = (int)T[512];
+    // forward: u states at segment starts
+    double2 ux = make_double2(1.0, 0.0);
+    double2 uy = cscale(1.0 / to, SL);  // u_{-1}; t_{-1} is an odd link
+    int ue = 0;
+    for (int s = 0; s < P; ++s) {
+        const double* T = lp.Tm + (item0 + s) * 9 * 64 + lane;
+        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
+        S[0 * 64] = ux.x;
+        S[1 * 64] = ux.y;
+        S[2 * 64] = uy.x;
+        S[3 * 64] = uy.y;
+        S[4 * 64] = (double)ue;
+        const double2 t00 = make_double2(T[0], T[64]), t01 = make_double2(T[128], T[192]);
+        const double2 t10 = make_double2(T[256], T[320]), t11 = make_double2(T[384], T[448]);
+        const int tE = (int)T[512];
+        const double2 nx = cadd(cmul(t00, ux), cmul(t01, uy));
+        const double2 ny = cadd(cmul(t10, ux), cmul(t11, uy));
+        ux = nx;
+        uy = ny;
+        ue += tE;
+        renorm(ux, uy, ue);
+    }
+    const double2 Dm = csub(cscale(tau, ux), cmul(SR, uy));  // D = Dm 2^ue
+    const int eN = ue;
 
-    // prefix: Pre_s = T_s ... T_0
-    double2 m00 = t00, m01 = t01, m10 = t10, m11 = t11;
-    int eT = tE;
-    for (int d = 1; d < P; d <<= 1) {
-        double2 o00 = make_double2(__shfl_up(m00.x, d, P), __shfl_up(m00.y, d, P));
-        double2 o01 = make_double2(__shfl_up(m01.x, d, P), __shfl_up(m01.y, d, P));
-        double2 o10 = make_double2(__shfl_up(m10.x, d, P), __shfl_up(m10.y, d, P));
-        double2 o11 = make_double2(__shfl_up(m11.x, d, P), __shfl_up(m11.y, d, P));
-        int oe = __shfl_up(eT, d, P);
-        if (s >= d) {
-            double2 r00, r01, r10, r11;
-            mat_mul(m00, m01, m10, m11, o00, o01, o10, o11, r00, r01, r10, r11);
-            m00 = r00;
-            m01 = r01;
-            m10 = r10;
-            m11 = r11;
-            eT += oe;
-        }
+    // backward: v states at segment ends, (v_{hi-1}, v_hi)
+    double2 vx = cscale(1.0 / tau, SR);  // v_N
+    double2 vy = make_double2(1.0, 0.0);  // v_{N-1}
+    int ve = 0;
+    for (int s = P - 1; s >= 0; --s) {
+        const double* T = lp.Tm + (item0 + s) * 9 * 64 + lane;
+        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
+        S[5 * 64] = vy.x;
+        S[6 * 64] = vy.y;
+        S[7 * 64] = vx.x;
+        S[8 * 64] = vx.y;
+        S[9 * 64] = (double)ve;
+        const int lo = min(N, s * L);
+        const int hi = min(N, lo + L);
+        if (lo >= hi) continue;  // empty tail segment: identity
+        const double2 t00 = make_double2(T[0], T[64]), t01 = make_double2(T[128], T[192]);
+        const double2 t10 = make_double2(T[256], T[320]), t11 = make_double2(T[384], T[448]);
+        const int tE = (int)T[512];
+        // det(T_s) = t_{lo-1} / t_{hi-1}; t_{lo-1} is odd (lo even)
+        const double t_hm1 = ((hi - 1) & 1) ? to : te;
+        const double idet = t_hm1 / to;
+        // (v_lo, v_{lo-1}) = adj(T) (v_hi, v_{hi-1}) 2^tE / det
+        const double2 nx = cscale(idet, csub(cmul(t11, vx), cmul(t01, vy)));
+        const double2 ny = cscale(idet, csub(cmul(t00, vy), cmul(t10, vx)));
+        vx = nx;
+        vy = ny;
+        ve += tE;
+        renorm(vx, vy, ve);
     }
-    const double2 sigL = cscale(1.0 / to, SL);  // u_{-1}; t_{-1} is an odd link
-    double2 wx = cadd(m00, cmul(m01, sigL));
-    double2 wy = cadd(m10, cmul(m11, sigL));
-    int we = eT;
-    renorm(wx, wy, we);
-    double2 ux = make_double2(__shfl_up(wx.x, 1, P), __shfl_up(wx.y, 1, P));
-    double2 uy = make_double2(__shfl_up(wy.x, 1, P), __shfl_up(wy.y, 1, P));
-    int ue = __shfl_up(we, 1, P);
-    if (s == 0) {
-        ux = make_double2(1.0, 0.0);
-        uy = sigL;
-        ue = 0;
-    }
-    const double2 uN = make_double2(__shfl(wx.x, P - 1, P), __shfl(wx.y, P - 1, P));
-    const double2 uN1 = make_double2(__shfl(wy.x, P - 1, P), __shfl(wy.y, P - 1, P));
-    const int eN = __shfl(we, P - 1, P);
-    const double2 Dm = csub(cscale(tau, uN), cmul(SR, uN1));  // D = Dm 2^eN
-
-    // suffix: Suf_s = T_{P-1} ... T_s
-    m00 = t00;
-    m01 = t01;
-    m10 = t10;
-    m11 = t11;
-    eT = tE;
-    for (int d = 1; d < P; d <<= 1) {
-        double2 o00 = make_double2(__shfl_down(m00.x, d, P), __shfl_down(m00.y, d, P));
-        double2 o01 = make_double2(__shfl_down(m01.x, d, P), __shfl_down(m01.y, d, P));
-        double2 o10 = make_double2(__shfl_down(m10.x, d, P), __shfl_down(m10.y, d, P));
-        double2 o11 = make_double2(__shfl_down(m11.x, d, P), __shfl_down(m11.y, d, P));
-        int oe = __shfl_down(eT, d, P);
-        if (s + d < P) {
-            double2 r00, r01, r10, r11;
-            mat_mul(o00, o01, o10, o11, m00, m01, m10, m11, r00, r01, r10, r11);
-            m00 = r00;
-            m01 = r01;
-            m10 = r10;
-            m11 = r11;
-            eT += oe;
-        }
-    }
-    const double2 sigR = cscale(1.0 / tau, SR);
-    double2 zx, zy;
-    int ze;
-    {
-        // Suf^{-1} = adj(M) 2^eT / det(T product), det = t_{lo-1}/t_{N-1}
-        double id = (lo < N) ? (tau / to) : 1.0;
-        zx = cscale(id, csub(cmul(m11, sigR), m01));
-        zy = cscale(id, csub(m00, cmul(m10, sigR)));
-        ze = eT;
-        renorm(zx, zy, ze);
-    }
-    double2 ax = make_double2(__shfl_down(zx.x, 1, P), __shfl_down(zx.y, 1, P));
-    double2 ay = make_double2(__shfl_down(zy.x, 1, P), __shfl_down(zy.y, 1, P));
-    int ae = __shfl_down(ze, 1, P);
-    const int hi = min(N, lo + tube.L);
-    if (s == P - 1 || hi >= N) {
-        ax = sigR;  // (v_N, v_{N-1})
-        ay = make_double2(1.0, 0.0);
-        ae = 0;
-    }
-    // states: u = (u_lo, u_{lo-1}), v = (v_{hi-1}, v_hi).  Padding lanes of the
-    // last group carry the last chain's values so the sweep stays finite.
-    double* S = lp.St + (long long)item * 10 * 64 + lane;
-    S[0 * 64] = ux.x;
-    S[1 * 64] = ux.y;
-    S[2 * 64] = uy.x;
-    S[3 * 64] = uy.y;
-    S[4 * 64] = (double)ue;
-    S[5 * 64] = ay.x;
-    S[6 * 64] = ay.y;
-    S[7 * 64] = ax.x;
-    S[8 * 64] = ax.y;
-    S[9 * 64] = (double)ae;
-    if (s == 0 && valid) {
+    if (valid) {
         double* Dp = lp.Dd + (long long)chain * 3;
         Dp[0] = Dm.x;
         Dp[1] = Dm.y;
@@ -478,10 +464,12 @@ template <int K, int NS, int KIND, bool WRITE_G>
 __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 {
     __shared__ double red_all[4][K * 65];
+    __shared__ double us_all[4][NEGF_SEG_MAX];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int item = blockIdx.x * 4 + wv;
-    if (item >= lp.nitems) return;
+    const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    if (idx >= lp.nlist) return;
+    const int item = lp.item_list ? __builtin_amdgcn_readfirstlane(lp.item_list[idx]) : idx;
     double* red = red_all[wv];
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
@@ -491,7 +479,8 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
     const int lo = min(N, ic.s * tube.L);
     const int hi = min(N, lo + tube.L);
     const int nsub = (hi - lo + K - 1) / K;  // wave-uniform
-    const double* __restrict__ U = tube.U;
+    double* us = us_all[wv];
+    stage_segment(us, tube.U, lo, hi, lane);
     const double* S = lp.St + (long long)item * 10 * 64 + lane;
     double2 ux = make_double2(S[0], S[64]), uy = make_double2(S[128], S[192]);
     int ue = (int)S[256];
@@ -513,20 +502,20 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
         int e = ve0;
         ck.put(nsub - 1, x, y, e);
         for (int j = nsub - 1; j >= 1; --j) {
-            const int p0 = lo + j * K;
-            const int len = min(K, hi - p0);
+            const int q0 = j * K;
+            const int len = min(K, hi - lo - q0);
             if (len == K) {
 #pragma unroll
                 for (int i = K - 1; i >= 0; i -= 2) {
-                    bwd_step<1>(k, U[p0 + i], x, y);
-                    bwd_step<0>(k, U[p0 + i - 1], x, y);
+                    bwd_step<1>(k, us[q0 + i], x, y);
+                    bwd_step<0>(k, us[q0 + i - 1], x, y);
                 }
             } else {
                 for (int i = len - 1; i >= 0; --i) {
                     if (i & 1)
-                        bwd_step<1>(k, U[p0 + i], x, y);
+                        bwd_step<1>(k, us[q0 + i], x, y);
                     else
-                        bwd_step<0>(k, U[p0 + i], x, y);
+                        bwd_step<0>(k, us[q0 + i], x, y);
                 }
             }
             renorm(x, y, e);
@@ -546,7 +535,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
         double2 vr[K];
         double Us[K];
 #pragma unroll
-        for (int i = 0; i < K; ++i) Us[i] = (i < len) ? U[p0 + i] : 0.0;
+        for (int i = 0; i < K; ++i) Us[i] = (i < len) ? us[p0 - lo + i] : 0.0;
         {
             double2 x = cx, y = cy;  // (v_{q-1}, v_q)
             if (len == K) {
@@ -609,6 +598,103 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
         }
     }
     if (KIND == KIND_DOS) lp.dos_part[(long long)item * 64 + lane] = dos_acc;
+}
+
+// Equilibrium sweep of a full segment of NS sub-blocks (the common case),
+// with the chain coefficient folded into the left state: with
+//   ut = (coef / D) 2^(e_u + e_v - e_D) u
+// (a solution of the same linear recurrence) the contribution of site n is
+// Re(ut_n v_n), two instructions.  Between sub-blocks ut only changes by the
+// exact power of two of the next v checkpoint, so u needs no renormalisation:
+// |ut_n v_n| = |coef G_nn| stays bounded.  Checkpoint indices are static.
+template <int K, int NS>
+__global__ __launch_bounds__(256, 2) void sweep_eq_full_kernel(WLaunch lp)
+{
+    __shared__ double red_all[4][K * 65];
+    __shared__ double us_all[4][K * NS];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    if (idx >= lp.nlist) return;
+    const int item = __builtin_amdgcn_readfirstlane(lp.item_list[idx]);
+    double* red = red_all[wv];
+    WTube tube;
+    ItemCtx ic = item_ctx(lp, item, lane, tube);
+    const ChainDesc ch = lp.chains[ic.chain];
+    const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
+    const int N = tube.N;
+    const int lo = ic.s * tube.L;  // the segment is [lo, lo + K NS)
+    double* us = us_all[wv];
+    stage_segment(us, tube.U, lo, lo + K * NS, lane);
+    const double* S = lp.St + (long long)item * 10 * 64 + lane;
+    double2 ux = make_double2(S[0], S[64]), uy = make_double2(S[128], S[192]);
+    const int ue = (int)S[256];
+    double2 x = make_double2(S[320], S[384]), y = make_double2(S[448], S[512]);
+    int e = (int)S[576];
+    const double* Dp = lp.Dd + (long long)ic.chain * 3;
+    const double2 iD = crecip(make_double2(Dp[0], Dp[1]));
+    const int eN = (int)Dp[2];
+    const double vmask = ic.valid ? 1.0 : 0.0;
+    const double2 cD = cmul(make_double2(ch.cr * vmask, ch.ci * vmask), iD);
+    double* prow = lp.part + tube.part_off + (long long)ic.g * N + lo;
+
+    // pass 2: v checkpoints at the right end of every sub-block
+    double2 cx[NS], cy[NS];
+    int ce[NS];
+    cx[NS - 1] = x;
+    cy[NS - 1] = y;
+    ce[NS - 1] = e;
+#pragma unroll
+    for (int j = NS - 1; j >= 1; --j) {
+#pragma unroll
+        for (int i = K - 1; i >= 1; i -= 2) {
+            bwd_step<1>(k, us[j * K + i], x, y);
+            bwd_step<0>(k, us[j * K + i - 1], x, y);
+        }
+        renorm(x, y, e);
+        cx[j - 1] = x;
+        cy[j - 1] = y;
+        ce[j - 1] = e;
+    }
+
+    // pass 3
+    {
+        const double sc = __builtin_amdgcn_ldexp(1.0, ue + ce[0] - eN);
+        const double2 C0 = cscale(sc, cD);
+        ux = cmul(C0, ux);
+        uy = cmul(C0, uy);
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double2 vr[K];
+        {
+            double2 xx = cx[j], yy = cy[j];
+            vr[K - 1] = xx;
+#pragma unroll
+            for (int i = K - 1; i >= 1; --i) {
+                if (i & 1)
+                    bwd_step<1>(k, us[j * K + i], xx, yy);
+                else
+                    bwd_step<0>(k, us[j * K + i], xx, yy);
+                vr[i - 1] = xx;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            red[i * 65 + lane] = fma(ux.x, vr[i].x, -ux.y * vr[i].y);
+            if (i & 1)
+                fwd_step<1>(k, us[j * K + i], ux, uy);
+            else
+                fwd_step<0>(k, us[j * K + i], ux, uy);
+        }
+        const double r = lane_sum_transpose<K>(red, lane);
+        if (lane < K) prow[j * K + lane] = r;
+        if (j + 1 < NS) {
+            const int d = ce[j + 1] - ce[j];
+            scale2(ux, d);
+            scale2(uy, d);
+        }
+    }
 }
 
 // Lead self-energies of every chain, Sigma = tau g tau^+ with tau = t_odd

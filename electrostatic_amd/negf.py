@@ -140,6 +140,12 @@ class Tube:
         n = self.lib.negf_tube_noneq_integrand(self.h, n, _dp(E), _dp(I))
         return E[:n], I[:3 * n].reshape(3, n)
 
+    def current(self):
+        """Compute_Current: terminal currents (A) at contacts 0 and 1."""
+        I = np.zeros(2)
+        _lib.check(self.lib.negf_tube_current(self.h, _dp(I)))
+        return I
+
     def dos(self, E_real):
         E = np.ascontiguousarray(E_real, np.float64)
         d = np.zeros(E.size)
@@ -259,3 +265,42 @@ def write_step_files(prefix, ptd, Q=None, U=None, norm=None):
     arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (Q, U, norm)]
     ptrs = [None if a is None else _dp(a) for a in arrs]
     _lib.check(lib.negf_write_step_files(prefix.encode(), ptd.size, _dp(ptd), *ptrs))
+
+
+def write_qin(prefix, ptd, Qin):
+    lib = _lib.load()
+    ptd = np.ascontiguousarray(ptd, np.float64)
+    q = np.ascontiguousarray(Qin, np.float64)
+    _lib.check(lib.negf_write_qin(prefix.encode(), ptd.size, _dp(ptd), _dp(q)))
+
+
+def write_charge_components(filename, ptd, eq, neq, rho0, Q):
+    lib = _lib.load()
+    a = [np.ascontiguousarray(x, np.float64) for x in (ptd, eq, neq, rho0, Q)]
+    _lib.check(lib.negf_write_charge_components(filename.encode(), a[0].size, *[_dp(x) for x in a]))
+
+
+def write_current_header(filename):
+    _lib.check(_lib.load().negf_write_current_header(filename.encode()))
+
+
+def write_current_line(filename, step, Vds, Vgs, I, avg_intg_pts, total_iterations, conductance):
+    I = np.ascontiguousarray(I, np.float64)
+    _lib.check(_lib.load().negf_write_current_line(filename.encode(), step, Vds, Vgs, _dp(I),
+                                                   avg_intg_pts, total_iterations, conductance))
+
+
+def read_charge_distribution(filename, N):
+    Q = np.zeros(N)
+    _lib.check(_lib.load().negf_read_charge_distribution(filename.encode(), N, _dp(Q)))
+    return Q
+
+
+def conductance(E, T, mu0, kT0):
+    """Compute_Conductance: per-energy conductance and the summed total."""
+    E = np.ascontiguousarray(E, np.float64)
+    T = np.ascontiguousarray(T, np.float64)
+    cond = np.zeros(E.size)
+    tot = C.c_double()
+    _lib.check(_lib.load().negf_conductance(E.size, _dp(E), _dp(T), mu0, kT0, _dp(cond), C.byref(tot)))
+    return cond, tot.value

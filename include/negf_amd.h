@@ -196,6 +196,10 @@ int negf_tube_noneq_layout(const negf_tube *t, int *npaths, int *pts,
  * required size if max_pts is smaller. */
 int negf_tube_noneq_integrand(const negf_tube *t, int max_pts, double *E_real,
                               double *integrand);
+/* Compute_Current (NEGF_Common.cpp:4321-4760) at the present potential over
+ * the non-equilibrium path: I[k] in A at contact k (I[1] = -I[0]); zero in
+ * equilibrium. */
+int negf_tube_current(negf_tube *t, double *I);
 /* Compute_DensityOfStates over nE real energies (E_zPlus added):
  * DOS = spin * sum_modes degen * trace / (2 pi) / atoms_per_unitcell and
  * T = sum_modes degen * Gamma_L Gamma_R |G_0,N-1|^2 (host arrays). */
@@ -261,6 +265,30 @@ int negf_scf_kernel(const negf_scf *s, double *K);
 int negf_write_step_files(const char *prefix, int N, const double *ptd_nm,
                           const double *Q, const double *U,
                           const double *norm);
+
+/* <prefix>_Qin.dat (Write_InputInducedCharge NEGF_Common.cpp:3081-3092). */
+int negf_write_qin(const char *prefix, int N, const double *ptd_nm,
+                   const double *Qin);
+/* _chargeComp.dat (Write_ChargeComponents NEGF_Common.cpp:2999-3023):
+ * Im RhoEq, Re RhoNonEq, Im Rho0 and Q per site. */
+int negf_write_charge_components(const char *filename, int N,
+                                 const double *ptd_nm, const double *eq_im,
+                                 const double *noneq_re, const double *rho0_im,
+                                 const double *Q);
+/* I.dat header (Define_FileHeaderForCurrent NEGF_Common.cpp:73-82) and one
+ * appended row per step (Write_Current :4818-4840). */
+int negf_write_current_header(const char *filename);
+int negf_write_current_line(const char *filename, int step, double Vds,
+                            double Vgs, const double *I, double avg_intg_pts,
+                            int total_iterations, double conductance);
+/* Restart: read a charge distribution written as <prefix>_Qout.dat
+ * (Read_Table1D NEGF_Common.cpp:236-286; header + N rows required). */
+int negf_read_charge_distribution(const char *filename, int N, double *Q);
+/* Compute_Conductance (NEGF_Common.cpp:1873-1928): per-energy conductance
+ * -T dF/dE (units of 2q^2/h, finite differences on the given grid) and its
+ * sum times the reference's step E_0 - E_1. */
+int negf_conductance(int nE, const double *E, const double *T, double mu0,
+                     double kT0, double *cond, double *total);
 
 #ifdef __cplusplus
 }

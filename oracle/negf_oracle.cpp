@@ -22,6 +22,7 @@ const double kPi = 3.14159265358979323846;
 const double kKb_eVperK = 8.617333262e-5;
 const double kQe = 1.602176634e-19;
 const double kEp0 = 8.8541878128e-12;
+const double kHeVperHz = 4.135667696e-15; /* PhysConst::h_eVperHz */
 
 struct C {
     double re, im;
@@ -921,6 +922,63 @@ int orc_dos_transmission(const orc_tube *t, const double *U,
         tr_T = Tsum.re;
         dos[e] = t->spin_degen * tr_dos / atoms_per_unitcell;
         trans[e] = tr_T;
+    }
+    return st;
+}
+
+/* Compute_Current NEGF_Common.cpp:4321-4760 (non-equilibrium path, contact
+ * sites only): IF_k = Gamma_k f_k A_kk - Gamma_k Gn_kk with A = sum_j A^j,
+ * Gn = sum_j A^j f_j; I_k += s q/h Re(IF_k g w mul).  Zero in equilibrium
+ * (no non-equilibrium path). */
+int orc_current(const orc_tube *t, const double *U, const double Uc[2],
+                double I[2])
+{
+    I[0] = I[1] = 0.;
+    orc_limits L;
+    orc_energy_limits(t, t->mu, &L);
+    if (!L.flag_noneq) return 0;
+    int N = t->N;
+    std::vector<Path> paths = noneq_paths(t, L);
+    double cm = t->spin_degen * kQe / kHeVperHz;
+    int gcid[2] = {0, N - 1};
+    Work wk(N);
+    int st = 0;
+    for (int m = 0; m < t->nmodes; ++m) {
+        Chain ch = make_chain(N, t->beta[m], t->gamma);
+        for (const Path &path : paths)
+            for (size_t e = 0; e < path.E.size(); ++e) {
+                C E = path.E[e];
+                C sigma[2];
+                st |= fill_alpha(t, ch, U, Uc, E, wk.alpha.data(), sigma);
+                C fc[2];
+                for (int c = 0; c < 2; ++c)
+                    fc[c] = fermi(E - mk(t->mu[c], 0.), L.kT_contact[c]);
+                rgf(N, wk.alpha.data(), ch.Hb, ch.Hc, wk.Xt.data(), wk.X.data(),
+                    wk.Yt.data(), wk.Y.data());
+                C Gkk[2], Gam[2];
+                for (int k = 0; k < 2; ++k) {
+                    int kg = gcid[k];
+                    Gkk[k] = mk(1., 0.) / (wk.alpha[kg] - wk.X[kg] - wk.Y[kg]);
+                    Gam[k] = mk(0., 1.) * (sigma[k] - conj(sigma[k]));
+                }
+                for (int kk = 0; kk < 2; ++kk) {
+                    int n = gcid[kk];
+                    C A = mk(0., 0.), Gn = mk(0., 0.);
+                    for (int k = 0; k < 2; ++k) {
+                        int kg = gcid[k];
+                        C temp = Gkk[k];
+                        for (int q = kg; q < n; ++q) temp = (-1. * wk.Xt[q]) * temp;
+                        for (int q = kg; q > n; q--) temp = (-1. * wk.Yt[q]) * temp;
+                        C A_nn = temp * Gam[k] * conj(temp);
+                        A = A + A_nn;
+                        Gn = Gn + A_nn * fc[k];
+                    }
+                    C IF = Gam[kk] * fc[kk] * A - Gam[kk] * Gn;
+                    C cur = (mk(cm, 0.) * IF * double(t->degen[m])) *
+                            mk(path.w[e], 0.) * path.mul[e];
+                    I[kk] += cur.re;
+                }
+            }
     }
     return st;
 }

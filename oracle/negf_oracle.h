@@ -113,6 +113,10 @@ int orc_accumulate_noneq(int N, const double *U, double t_even, double t_odd,
                          const double *coefL, const double *coefR,
                          double *acc_re);
 
+/* ---- terminal current (post-process) ---- */
+int orc_current(const orc_tube *t, const double *U, const double Uc[2],
+                double I[2]);
+
 /* ---- DOS / transmission (post-process) ---- */
 int orc_dos_transmission(const orc_tube *t, const double *U,
                          const double Uc[2], int nE, const orc_c *E,
@@ -158,6 +162,16 @@ int orc_scf_bias_point(const orc_tube *t, const orc_c *Rho0, const double *K,
 /* ---- per-step output files (reference iostream formatting) ---- */
 void orc_write_step_files(const char *prefix, int N, const double *ptd,
                           const double *Q, const double *U, const double *norm);
+void orc_write_qin(const char *prefix, int N, const double *ptd,
+                   const double *Qin);
+void orc_write_charge_components(const char *filename, int N,
+                                 const double *ptd, const double *eq,
+                                 const double *neq, const double *r0,
+                                 const double *Q);
+void orc_write_current_header(const char *filename);
+void orc_write_current_line(const char *filename, int step, double Vds,
+                            double Vgs, const double *I, double avg_pts,
+                            int iters, double conductance);
 
 #ifdef __cplusplus
 }

@@ -449,3 +449,33 @@ def write_step_files(prefix, ptd, Q=None, U=None, norm=None):
     arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (Q, U, norm)]
     lib().orc_write_step_files(prefix.encode(), ptd.size, _dp(ptd),
                                *[None if a is None else _dp(a) for a in arrs])
+
+
+def current(t, U, Uc):
+    U = np.ascontiguousarray(U, np.float64)
+    Uc = np.ascontiguousarray(Uc, np.float64)
+    I = np.zeros(2)
+    st = lib().orc_current(C.byref(t), _dp(U), _dp(Uc), _dp(I))
+    return I, st
+
+
+def write_qin(prefix, ptd, Qin):
+    ptd = np.ascontiguousarray(ptd, np.float64)
+    q = np.ascontiguousarray(Qin, np.float64)
+    lib().orc_write_qin(prefix.encode(), ptd.size, _dp(ptd), _dp(q))
+
+
+def write_charge_components(filename, ptd, eq, neq, r0, Q):
+    a = [np.ascontiguousarray(x, np.float64) for x in (ptd, eq, neq, r0, Q)]
+    lib().orc_write_charge_components(filename.encode(), a[0].size, *[_dp(x) for x in a])
+
+
+def write_current_header(filename):
+    lib().orc_write_current_header(filename.encode())
+
+
+def write_current_line(filename, step, Vds, Vgs, I, avg_pts, iters, conductance):
+    I = np.ascontiguousarray(I, np.float64)
+    lib().orc_write_current_line(filename.encode(), C.c_int(step), C.c_double(Vds),
+                                 C.c_double(Vgs), _dp(I), C.c_double(avg_pts),
+                                 C.c_int(iters), C.c_double(conductance))

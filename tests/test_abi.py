@@ -95,3 +95,55 @@ def test_step_files_match_reference_format(tmp_path):
     lines = open(a + "_Qout.dat").read().splitlines()
     assert lines[0] == "'axial location / (nm)', 'Induced charge per site / (e)'"
     assert len(lines) == N + 1 and len(lines[1]) == 70
+
+
+def test_qin_and_charge_components_match_reference_format(tmp_path):
+    rng = np.random.default_rng(5)
+    N = 21
+    ptd = np.linspace(-3.2, 3.2, N)
+    q = rng.normal(size=N) * 1e-3
+    comps = [rng.normal(size=N) for _ in range(3)]
+    a, b = str(tmp_path / "p"), str(tmp_path / "o")
+    negf.write_qin(a, ptd, q)
+    O.write_qin(b, ptd, q)
+    assert open(a + "_Qin.dat", "rb").read() == open(b + "_Qin.dat", "rb").read()
+    negf.write_charge_components(a + "_chargeComp.dat", ptd, *comps, q)
+    O.write_charge_components(b + "_chargeComp.dat", ptd, *comps, q)
+    assert open(a + "_chargeComp.dat", "rb").read() == open(b + "_chargeComp.dat", "rb").read()
+
+
+def test_current_file_matches_reference_format(tmp_path):
+    a, b = str(tmp_path / "I_p.dat"), str(tmp_path / "I_o.dat")
+    for f, mod in ((a, negf), (b, O)):
+        mod.write_current_header(f)
+        mod.write_current_line(f, 3, 0.1, 0.5, np.array([1.234e-7, -1.234e-7]), 212.5, 17, 0.731)
+        mod.write_current_line(f, 4, 0.2, 0.5, np.array([2.5e-6, -2.5e-6]), 200.0, 9, 1.25)
+    assert open(a, "rb").read() == open(b, "rb").read()
+
+
+def test_restart_reads_qout(tmp_path):
+    rng = np.random.default_rng(6)
+    N = 40
+    ptd = np.linspace(-5, 5, N)
+    Q = rng.normal(size=N) * 1e-3
+    pre = str(tmp_path / "step0007")
+    negf.write_step_files(pre, ptd, Q=Q)
+    back = negf.read_charge_distribution(pre + "_Qout.dat", N)
+    # the file carries 15 significant digits, as the reference writes it
+    assert np.abs(back - Q).max() <= 1e-14 * np.abs(Q).max()
+    with pytest.raises(_lib.NegfError):
+        negf.read_charge_distribution(pre + "_Qout.dat", N + 1)
+
+
+def test_conductance_of_a_perfect_channel():
+    # T = 1 over a grid spanning mu +- 30 kT: the two sign flips of the
+    # reference (dE = E_0 - E_1 < 0 in both the difference and the sum)
+    # cancel and the total is the integral of -dF/dE = 1
+    kT = 8.617333262e-5 * 298.0
+    E = np.linspace(-1.0 - 30 * kT, -1.0 + 30 * kT, 2001)
+    cond, tot = negf.conductance(E, np.ones_like(E), -1.0, kT)
+    assert abs(tot - 1.0) < 1e-5
+    assert np.all(cond <= 0.0)                # -T dF/dE with dE < 0
+    # two conducting modes double it
+    _, tot2 = negf.conductance(E, 2 * np.ones_like(E), -1.0, kT)
+    assert abs(tot2 - 2 * tot) < 1e-14

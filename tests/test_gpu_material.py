@@ -249,3 +249,16 @@ def test_scf_nonequilibrium_bias_point():
     Uo, Qo, ito, nrmo, st = O.scf_bias_point(ot, r0, K, 0.2, ob)
     assert st == 0
     assert np.abs(U - Uo).max() < U_TOL
+
+
+def test_current_matches_oracle():
+    mu = (-1.0, -1.2)
+    tube, ot = pair(num_unitcells=30, num_modes=2, mu=mu)
+    U = gaussian_bumps(tube.N, seed=12, amp=0.2)
+    Uc = (U[0], U[-1])
+    tube.set_potential(U, Uc)
+    tube.solve(1)
+    I = tube.current()
+    Io, st = O.current(ot, U, Uc)
+    assert np.abs(I - Io).max() < 1e-10 * np.abs(Io).max()
+    assert I[0] > 0 and abs(I[0] + I[1]) < 1e-12 * abs(I[0])

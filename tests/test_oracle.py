@@ -358,3 +358,37 @@ def test_broyden_reset_keeps_or_resets_input():
     assert np.allclose(b.get_nin(), kept) and b.step_count == 1
     b.reset(0, 0.5)
     assert np.allclose(b.get_nin(), 0.5)
+
+
+# ---------------------------------------------------------------- current
+def test_current_is_landauer():
+    # the reference's contact-site form (Gamma_k f_k A_kk - Gamma_k Gn_kk)
+    # equals Landauer s q/h sum w T (f_L - f_R), and I_1 = -I_0
+    mu = (-1.0, -1.2)
+    N = 24
+    rng = np.random.default_rng(4)
+    U = rng.uniform(-0.2, 0.2, N)
+    t = O.make_tube(N, [BETA6], [2], mu=mu, noneq_pts=(150,), eta=1e-5)
+    I, st = O.current(t, U, (U[0], U[-1]))
+    assert abs(I[0] + I[1]) < 1e-12 * abs(I[0])
+    L = O.energy_limits(t, mu)
+    a = complex(L.E_contour_right.re, L.E_contour_right.im)
+    b = complex(L.E_rightmost.re, L.E_rightmost.im)
+    E, w, mul = O.gl_path(a, b, 150, 0)
+    dos, tr, _ = O.dos_transmission(t, U, (U[0], U[-1]), E, 68)
+    kT = [8.617333262e-5 * 298.0] * 2
+    fL = np.array([O.fermi(e - mu[0], kT[0]) for e in E])
+    fR = np.array([O.fermi(e - mu[1], kT[1]) for e in E])
+    land = 2 * 1.602176634e-19 / 4.135667696e-15 * np.sum(w * tr * ((fL - fR) * mul).real)
+    assert abs(I[0] - land) < 1e-10 * abs(land)
+    # a clean chain at small bias conducts g = 2 modes: I ~ (2 s q/h) dmu
+    t0 = O.make_tube(N, [BETA6], [2], mu=(-1.0, -1.01), noneq_pts=(400,), eta=1e-5)
+    I0, _ = O.current(t0, np.zeros(N), (0.0, 0.0))
+    G0 = 2 * 2 * 1.602176634e-19 / 4.135667696e-15
+    assert abs(I0[0] / (G0 * 0.01) - 1.0) < 1e-3
+
+
+def test_current_is_zero_in_equilibrium():
+    t = O.make_tube(12, [BETA6], [2], mu=(-1.0, -1.0))
+    I, _ = O.current(t, np.zeros(12), (0.0, 0.0))
+    assert I[0] == 0.0 and I[1] == 0.0
