@@ -116,6 +116,8 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--segment-length", type=int, default=0,
+                    help="RGF segment length (0: library default)")
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the secondary SCF gate-sweep measurement")
     ap.add_argument("--write-diag-g", action="store_true",
@@ -147,6 +149,7 @@ def main():
     p.t_even = wl["t_even"]
     p.t_odd = wl["t_odd"]
     p.Uc[0], p.Uc[1] = wl["Uc"]
+    p.segment_length = args.segment_length
     plan = C.c_void_p()
     _lib.check(lib.negf_rgf_plan_create(p, C.c_void_p(dU.data_ptr()), nE,
                                         C.c_void_p(dE.data_ptr()),
@@ -217,6 +220,7 @@ def main():
             "N": N,
             "energies": nE,
             "energy_sites": energy_sites,
+            "segment_length": args.segment_length or 128,
         },
         "roofline": {
             "bound": "mfma",

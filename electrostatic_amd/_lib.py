@@ -30,7 +30,8 @@ class negf_chain_params(C.Structure):
                 ("Uc", C.c_double * 2), ("use_decimation", C.c_int),
                 ("decimation_layers", C.c_int),
                 ("decimation_max_iter", C.c_int),
-                ("decimation_rel_error", C.c_int)]
+                ("decimation_rel_error", C.c_int),
+                ("segment_length", C.c_int)]
 
 
 class negf_tube_params(C.Structure):
@@ -58,6 +59,7 @@ class negf_tube_params(C.Structure):
         ("integrand_correction_interval", C.c_int),
         ("kT_window_around_singularity", C.c_double * 2),
         ("flatband_dos_integration_pts", C.c_int),
+        ("segment_length", C.c_int),
     ]
 
 
@@ -109,6 +111,17 @@ SIGNATURES = {
     "negf_tube_noneq_layout": (C.c_int, [vp, ip, ip, dp]),
     "negf_tube_dos": (C.c_int, [vp, C.c_int, dp, dp, dp]),
     "negf_tube_current": (C.c_int, [vp, dp]),
+    "negf_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+    "negf_batch_destroy": (C.c_int, [vp]),
+    "negf_batch_offsets": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+    "negf_batch_solve": (C.c_int, [vp, C.c_int, vp, vp]),
+    "negf_batch_use_graph": (C.c_int, [vp, C.c_int]),
+    "negf_batch_set_segment_length": (C.c_int, [vp, C.c_int]),
+    "negf_batch_autotune": (C.c_int, [vp, vp, ip, dp]),
+    "negf_batch_enable_timing": (C.c_int, [vp, C.c_int]),
+    "negf_batch_last_time": (C.c_int, [vp, dp]),
+    "negf_batch_energy_sites": (C.c_int, [vp, dp]),
+    "negf_batch_status": (C.c_int, [vp]),
     "negf_broyden_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]),
     "negf_broyden_destroy": (C.c_int, [vp]),
     "negf_broyden_set_nin": (C.c_int, [vp, dp]),

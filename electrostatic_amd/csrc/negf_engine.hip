@@ -70,18 +70,20 @@ void HostBuf::release()
 }
 
 // ---------------------------------------------------------------- segmentation
-int choose_segments(int N, int* P, int* L)
+// Segments of L sites (a multiple of the sub-block K, at most NEGF_SEG_MAX);
+// all but the last are full.  L_req = 0 picks the default.
+int choose_segments(int N, int L_req, int* P, int* L)
 {
-    for (int p = 1; p <= 64; p <<= 1) {
-        int l0 = (N + p - 1) / p;
-        int l = (l0 + NEGF_SUB_K - 1) / NEGF_SUB_K * NEGF_SUB_K;
-        if (l <= NEGF_SEG_MAX) {
-            *P = p;
-            *L = l;
-            return 1;
-        }
-    }
-    return 0;
+    if (N < 1) return 0;
+    int l = L_req > 0 ? L_req : NEGF_SEG_DEFAULT;
+    l = (l + NEGF_SUB_K - 1) / NEGF_SUB_K * NEGF_SUB_K;
+    l = std::min(std::max(l, NEGF_SUB_K), NEGF_SEG_MAX);
+    l = std::min(l, (N + NEGF_SUB_K - 1) / NEGF_SUB_K * NEGF_SUB_K);
+    const long long p = (N + l - 1) / l;
+    if (p > NEGF_MAX_SEGMENTS) return 0;
+    *P = (int)p;
+    *L = l;
+    return 1;
 }
 int max_sites() { return NEGF_MAX_SITES; }
 
@@ -178,7 +180,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
             return NEGF_EARG;
         }
         int P, L;
-        if (!choose_segments(tb.N, &P, &L)) {
+        if (!choose_segments(tb.N, tb.L, &P, &L)) {
             set_error("chain of %d sites exceeds the supported maximum %d", tb.N, max_sites());
             return NEGF_ENOTSUP;
         }
@@ -320,7 +322,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     }
     const int nblk = (nitems_ + 3) / 4;
     hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3(nblk), dim3(256), 0, st, lp);
-    hipLaunchKernelGGL(scan_kernel, dim3((nsblk_ + 3) / 4), dim3(256), 0, st, lp, nsblk_);
+    hipLaunchKernelGGL(scan_kernel, dim3(nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
     for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
         if (fast_n_[ns] == 0) continue;
         WLaunch lf = lp;
@@ -354,14 +356,10 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         return NEGF_OK;
     }
     // fixed-order reduction of the group rows
-    dim3 g1((max_N_ + 255) / 256, max_chunks_, ntubes_);
-    hipLaunchKernelGGL(reduce_rows_kernel, g1, dim3(256), 0, st, d_part_.as<double>(),
-                       d_red_src_off_.as<long long>(), d_red_nrows_.as<int>(), d_red_N_.as<int>(), reduce_chunk_,
-                       d_tmp_.as<double>(), d_red_dst_off_.as<long long>());
-    dim3 g2((max_N_ + 255) / 256, 1, ntubes_);
-    hipLaunchKernelGGL(reduce_rows_kernel, g2, dim3(256), 0, st, d_tmp_.as<double>(),
-                       d_red2_src_off_.as<long long>(), d_red2_nrows_.as<int>(), d_red_N_.as<int>(), 1 << 30,
-                       d_out, d_red2_dst_off_.as<long long>());
+    dim3 g1((max_N_ + 15) / 16, ntubes_);
+    hipLaunchKernelGGL(reduce_rows1_kernel, g1, dim3(256), 0, st, d_part_.as<double>(),
+                       d_red_src_off_.as<long long>(), d_red_nrows_.as<int>(), d_red_N_.as<int>(), d_out,
+                       d_red2_dst_off_.as<long long>());
     NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }
@@ -455,6 +453,7 @@ static RgfTube one_tube(const negf_chain_params* p, const double* dU, int nE)
     tb.chain0 = 0;
     tb.nchain = nE;
     tb.out_off = 0;
+    tb.L = p->segment_length;
     // else-if order of the reference (NEGF_Common.cpp:3457-3495): a site
     // that matches an earlier sample is not sampled again
     tb.sample[0] = p->N / 2;

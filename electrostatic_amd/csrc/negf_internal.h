@@ -64,6 +64,7 @@ struct RgfTube {
     const double* dUc = nullptr;  // or: device contact potentials (2 values)
     double t_odd = 0;             // gamma
     int sample[3] = {-1, -1, -1}; // integrand sample sites (KIND_NONEQ)
+    int L = 0;                    // requested segment length (0: default)
     int chain0 = 0, nchain = 0;   // chains [chain0, chain0+nchain) of d_chains
     long long out_off = 0;        // offset of this tube's N outputs in d_out
 };
@@ -113,9 +114,9 @@ class RgfPlan {
 void gauss_legendre(int n, double* x, double* w);
 void gl_path(double ar, double ai, double br, double bi, int n, int type, double* E, double* w, double* mul);
 
-// Segmentation of a chain of N sites: P segments (power of two <= 64) of L
-// sites (multiple of NEGF_SUB_K, at most NEGF_SEG_MAX); 0 if N is too long.
-int choose_segments(int N, int* P, int* L);
+// Segmentation of a chain of N sites into P segments of L sites (L a
+// multiple of NEGF_SUB_K, at most NEGF_SEG_MAX); 0 if N is too long.
+int choose_segments(int N, int L_req, int* P, int* L);
 int max_sites();
 
 }  // namespace negf

@@ -34,6 +34,8 @@ class Tube {
     // Copy a device potential; the contact potentials become U[0], U[N-1].
     int set_potential_device(const double* dU, hipStream_t st);
     void set_contacts(const double mu[2], const double T[2]);
+    // Host part of a solve: limits and paths (plans rebuilt if needed).
+    int prepare(int iter);
     // Enqueue one solve; Q (device, N) is ready on stream st when it returns.
     int solve(int iter, double* dQ, hipStream_t st);
     // Synchronise; record rootless surface GFs of the last solves.
@@ -62,6 +64,8 @@ class Tube {
     double E_at_max = 0.0;
     Path neq_path;
     std::vector<double> integrand;   // [3][n_neq_E] after an adaptive update
+    std::vector<ChainDesc> eq_host, neq_host;   // chain lists of the current paths
+    long long paths_version = 0;                // bumped at every path rebuild
     int n_eq = 0, n_eq_pts = 0, n_neq = 0, n_neq_E = 0;
 
     hipStream_t stream = nullptr;
@@ -72,9 +76,12 @@ class Tube {
     HostBuf h_stage;
     RgfPlan eq_plan, neq_plan;
 
-   private:
-    RgfTube rgf_tube(const double* dU, const double* dUc, int nchain) const;
+    // RGF tube descriptor of this tube; seg_len < 0: the tube's own setting
+    RgfTube rgf_tube(const double* dU, const double* dUc, int nchain, int seg_len = -1) const;
+
     DecParams dec() const;
+
+   private:
     int upload_chains(DevBuf& buf, const std::vector<ChainDesc>& ch);
     void eq_chains_of(const Path& path, double cm, bool with_fermi, std::vector<ChainDesc>& out) const;
     int compute_rho0();

@@ -229,7 +229,7 @@ int Tube::init(const negf_tube_params& prm)
     }
     N = 4 * p.num_unitcells;   // rings_per_unitcell * num_unitcells (CNT.H:74)
     int P, Lseg;
-    if (!choose_segments(N, &P, &Lseg)) {
+    if (!choose_segments(N, p.segment_length, &P, &Lseg)) {
         set_error("tube of %d sites exceeds the supported maximum %d", N, max_sites());
         return NEGF_ENOTSUP;
     }
@@ -278,7 +278,7 @@ int Tube::init(const negf_tube_params& prm)
     return NEGF_OK;
 }
 
-RgfTube Tube::rgf_tube(const double* dU, const double* dUc, int nchain) const
+RgfTube Tube::rgf_tube(const double* dU, const double* dUc, int nchain, int seg_len) const
 {
     RgfTube tb;
     tb.N = N;
@@ -288,6 +288,7 @@ RgfTube Tube::rgf_tube(const double* dU, const double* dUc, int nchain) const
     tb.chain0 = 0;
     tb.nchain = nchain;
     tb.out_off = 0;
+    tb.L = seg_len >= 0 ? seg_len : p.segment_length;
     // else-if order of the reference (NEGF_Common.cpp:3469-3495)
     tb.sample[0] = N / 2;
     tb.sample[1] = N / 4;
@@ -396,6 +397,7 @@ int Tube::build_noneq_plan()
     n_neq_E = (int)nE;
     if (nE == 0) {
         n_neq = 0;
+        neq_host.clear();
         return NEGF_OK;
     }
     const double cm = -spin_degen / (2.0 * kPi);
@@ -422,6 +424,7 @@ int Tube::build_noneq_plan()
     int rc;
     if ((rc = upload_chains(neq_chains, ch))) return rc;
     n_neq = (int)ch.size();
+    neq_host = ch;
     if ((rc = d_integ.ensure(sizeof(double) * 3 * n_neq))) return rc;
     std::vector<RgfTube> tubes{rgf_tube(d_U.as<double>(), d_Uc.as<double>(), n_neq)};
     return neq_plan.build(KIND_NONEQ, false, tubes, neq_chains.as<ChainDesc>(), n_neq, dec());
@@ -516,6 +519,8 @@ int Tube::update_paths(bool correct)
     int rc;
     if ((rc = upload_chains(eq_chains, ch))) return rc;
     n_eq = (int)ch.size();
+    eq_host = ch;
+    ++paths_version;
     std::vector<RgfTube> tubes{rgf_tube(d_U.as<double>(), d_Uc.as<double>(), n_eq)};
     if ((rc = eq_plan.build(KIND_EQ, false, tubes, eq_chains.as<ChainDesc>(), n_eq, dec()))) return rc;
     if (L.noneq) {
@@ -528,6 +533,7 @@ int Tube::update_paths(bool correct)
     } else {
         n_neq = 0;
         n_neq_E = 0;
+        neq_host.clear();
     }
     return NEGF_OK;
 }
@@ -561,9 +567,22 @@ void Tube::set_contacts(const double m_[2], const double T_[2])
 }
 
 // Solve_NEGF (NEGF_Common.cpp:1309-1340) + Compute_InducedCharge (:2850-2900).
-int Tube::solve(int iter, double* dQ, hipStream_t st)
+// The host part of Solve_NEGF (:1314-1337): limits and paths when the
+// contacts changed or an adaptive correction falls due at this iteration.
+int Tube::prepare(int iter)
 {
     const bool correct = p.flag_adaptive_integration_limits && (iter % p.integrand_correction_interval == 0);
+    if (ec_updated) compute_limits(p, mu, T, L);
+    if (ec_updated || correct) {
+        int rc = update_paths(correct);
+        if (rc) return rc;
+        ec_updated = false;
+    }
+    return NEGF_OK;
+}
+
+int Tube::solve(int iter, double* dQ, hipStream_t st)
+{
     int rc;
     if (st != stream) {
         // order the tube's own stream after the caller's work (potential upload)
@@ -573,11 +592,7 @@ int Tube::solve(int iter, double* dQ, hipStream_t st)
         NEGF_HIP_TRY(hipStreamWaitEvent(stream, ev, 0));
         NEGF_HIP_TRY(hipEventDestroy(ev));
     }
-    if (ec_updated) compute_limits(p, mu, T, L);
-    if (ec_updated || correct) {
-        if ((rc = update_paths(correct))) return rc;
-        ec_updated = false;
-    }
+    if ((rc = prepare(iter))) return rc;
     if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, stream))) return rc;
     const bool have_neq = L.noneq && n_neq > 0;
     if (have_neq) {

@@ -6,7 +6,9 @@
 #define NEGF_SUB_NS 8                 // sub-blocks per segment (max)
 #define NEGF_SEG_MAX (NEGF_SUB_K * NEGF_SUB_NS)  // sites per segment (max)
 #define NEGF_MAX_DECIMATION_LAYERS 128
-#define NEGF_MAX_SITES (64 * NEGF_SEG_MAX)      // 64 segments of 128 sites
+#define NEGF_SEG_DEFAULT 128                     // default segment length
+#define NEGF_MAX_SEGMENTS 8192                   // segments per chain (scan is sequential)
+#define NEGF_MAX_SITES (NEGF_MAX_SEGMENTS * NEGF_SEG_MAX)
 
 namespace negf {
 

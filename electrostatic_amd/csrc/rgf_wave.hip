@@ -221,8 +221,11 @@ __device__ __forceinline__ void stage_segment(double* us, const double* __restri
     __builtin_amdgcn_wave_barrier();
 }
 
+// (256, 8): 8 workgroups of 4 waves per CU, i.e. 8 waves per SIMD, which
+// keeps the kernel at <= 64 VGPRs so a 128-group x 64-segment launch runs in
+// one round.
 template <int K>
-__global__ __launch_bounds__(256) void transfer_kernel(WLaunch lp)
+__global__ __launch_bounds__(256, 8) void transfer_kernel(WLaunch lp)
 {
     __shared__ double us_all[4][NEGF_SEG_MAX];
     const int lane = threadIdx.x & 63;
@@ -271,26 +274,85 @@ __global__ __launch_bounds__(256) void transfer_kernel(WLaunch lp)
 }
 
 // ------------------------------------------------------------ scan
-// One wavefront per 64-chain group, lane = chain: the segment states are
-// propagated through the P segment transfer matrices in order, so every read
-// of Tm and write of St is a coalesced 512-byte row.
+// Segment states.  With T_s the transfer matrix of segment s (sites
+// [lo, hi)) of the recurrence u_{n+1} = a_n u_n - c_n u_{n-1} on column
+// vectors (u_n, u_{n-1}):
 //   u: U_0 = (u_0, u_{-1}) = (1, Sigma_L/t_{-1});  U_hi = T_s U_lo
 //   v: the same recurrence read backwards, V_lo = T_s^{-1} V_hi with
-//      T_s^{-1} = adj(T_s) / det(T_s) and det(T_s) = t_{lo-1}/t_{hi-1}
+//      T_s^{-1} = adj(T_s) / det(T_s), det(T_s) = t_{lo-1}/t_{hi-1}
 //      (product of the per-site determinants c_n);
 //      V_N = (v_N, v_{N-1}) = (Sigma_R/t_{N-1}, 1)
 //   D = t_{N-1} u_N - Sigma_R u_{N-1}.
-__global__ __launch_bounds__(256) void scan_kernel(WLaunch lp, int ngroups)
+// One workgroup of SCAN_WAVES wavefronts per 64-chain group, lane = chain,
+// so every Tm read and St write is a coalesced row.  Wave w owns a contiguous
+// range of segments: it multiplies its transfer matrices (phase A), the
+// range products are exchanged through LDS and each wave forms its entry
+// states from the products of the ranges before (u) and after (v) it
+// (phase B), then it runs both states through its own range writing the
+// per-segment states (phase C).  The sequential depth is about 3 P /
+// SCAN_WAVES matrix steps instead of 2 P.
+#define SCAN_WAVES 16
+
+struct Mat2 {
+    double2 a, b, c, d;  // [[a, b], [c, d]]
+    int e;               // exponent: the matrix is (a, b, c, d) 2^e
+};
+
+__device__ __forceinline__ Mat2 load_T(const WLaunch& lp, long long item, int lane)
 {
+    const double* T = lp.Tm + item * 9 * 64 + lane;
+    Mat2 m;
+    m.a = make_double2(T[0], T[64]);
+    m.b = make_double2(T[128], T[192]);
+    m.c = make_double2(T[256], T[320]);
+    m.d = make_double2(T[384], T[448]);
+    m.e = (int)T[512];
+    return m;
+}
+
+// r = x * y
+__device__ __forceinline__ Mat2 mat2_mul(const Mat2& x, const Mat2& y)
+{
+    Mat2 r;
+    mat_mul(x.a, x.b, x.c, x.d, y.a, y.b, y.c, y.d, r.a, r.b, r.c, r.d);
+    r.e = x.e + y.e;
+    renorm_mat(r.a, r.b, r.c, r.d, r.e);
+    return r;
+}
+
+// (x, y) <- M (x, y)
+__device__ __forceinline__ void mat2_apply(const Mat2& m, double2& x, double2& y, int& ex)
+{
+    const double2 nx = cadd(cmul(m.a, x), cmul(m.b, y));
+    const double2 ny = cadd(cmul(m.c, x), cmul(m.d, y));
+    x = nx;
+    y = ny;
+    ex += m.e;
+    renorm(x, y, ex);
+}
+
+// (x, y) <- M^{-1} (x, y) = adj(M) (x, y) / det, idet = 1/det of the true matrix
+__device__ __forceinline__ void mat2_apply_inv(const Mat2& m, double idet, double2& x, double2& y, int& ex)
+{
+    const double2 nx = cscale(idet, csub(cmul(m.d, x), cmul(m.b, y)));
+    const double2 ny = cscale(idet, csub(cmul(m.a, y), cmul(m.c, x)));
+    x = nx;
+    y = ny;
+    ex += m.e;
+    renorm(x, y, ex);
+}
+
+__global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int ngroups)
+{
+    __shared__ double xm[SCAN_WAVES][9][64];
     const int lane = threadIdx.x & 63;
-    const int gi = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-    if (gi >= ngroups) return;
-    const int t = __builtin_amdgcn_readfirstlane(lp.sblk_tube[gi]);
-    const int g = __builtin_amdgcn_readfirstlane(lp.sblk_chain[gi]);
+    const int w = threadIdx.x >> 6;
+    const int gi = blockIdx.x;
+    if (gi >= ngroups) return;  // uniform over the block
+    const int t = lp.sblk_tube[gi];
+    const int g = lp.sblk_chain[gi];
     const WTube tube = lp.tubes[t];
-    const int P = __builtin_amdgcn_readfirstlane(tube.P);
-    const int L = __builtin_amdgcn_readfirstlane(tube.L);
-    const int N = __builtin_amdgcn_readfirstlane(tube.N);
+    const int P = tube.P, L = tube.L, N = tube.N;
     const int lc = g * 64 + lane;
     const bool valid = lc < tube.nchain;
     const int chain = tube.chain0 + (valid ? lc : tube.nchain - 1);
@@ -300,62 +362,94 @@ __global__ __launch_bounds__(256) void scan_kernel(WLaunch lp, int ngroups)
     const double2 SL = make_double2(ch.SLr, ch.SLi);
     const double2 SR = make_double2(ch.SRr, ch.SRi);
     const long long item0 = tube.item0 + (long long)g * P;
+    // segment range of each wave, and 1/det of a range [lo, hi) of sites
+    const int per = (P + SCAN_WAVES - 1) / SCAN_WAVES;
+    auto range_lo = [&](int k) { return min(P, k * per); };
+    auto inv_det = [&](int slo, int shi) {
+        // sites [slo*L, min(N, shi*L)); t_{lo-1} is an odd link (lo even)
+        const int hi = min(N, shi * L);
+        const int lo = min(N, slo * L);
+        if (lo >= hi) return 1.0;
+        const double t_hm1 = ((hi - 1) & 1) ? to : te;
+        return t_hm1 / to;
+    };
+    const int s0 = range_lo(w), s1 = range_lo(w + 1);
 
-    // forward: u states at segment starts
+    // phase A: M_w = T_{s1-1} ... T_{s0}
+    Mat2 M;
+    M.a = make_double2(1.0, 0.0);
+    M.b = make_double2(0.0, 0.0);
+    M.c = make_double2(0.0, 0.0);
+    M.d = make_double2(1.0, 0.0);
+    M.e = 0;
+    // (registers cannot hold the range for phase C at 1024 threads per block:
+    // phase C reloads the transfer matrices)
+    const bool cached = false;
+    Mat2 Tk[4];
+    for (int c = s0; c < s1; c += 4) {
+        // issue the loads of up to four segments before the products
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < s1) Tk[q] = load_T(lp, item0 + c + q, lane);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < s1) M = mat2_mul(Tk[q], M);
+    }
+    xm[w][0][lane] = M.a.x;
+    xm[w][1][lane] = M.a.y;
+    xm[w][2][lane] = M.b.x;
+    xm[w][3][lane] = M.b.y;
+    xm[w][4][lane] = M.c.x;
+    xm[w][5][lane] = M.c.y;
+    xm[w][6][lane] = M.d.x;
+    xm[w][7][lane] = M.d.y;
+    xm[w][8][lane] = (double)M.e;
+    __syncthreads();
+    auto get = [&](int k) {
+        Mat2 m;
+        m.a = make_double2(xm[k][0][lane], xm[k][1][lane]);
+        m.b = make_double2(xm[k][2][lane], xm[k][3][lane]);
+        m.c = make_double2(xm[k][4][lane], xm[k][5][lane]);
+        m.d = make_double2(xm[k][6][lane], xm[k][7][lane]);
+        m.e = (int)xm[k][8][lane];
+        return m;
+    };
+
+    // phase B: entry states of this wave's range
     double2 ux = make_double2(1.0, 0.0);
     double2 uy = cscale(1.0 / to, SL);  // u_{-1}; t_{-1} is an odd link
     int ue = 0;
-    for (int s = 0; s < P; ++s) {
-        const double* T = lp.Tm + (item0 + s) * 9 * 64 + lane;
-        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
-        S[0 * 64] = ux.x;
-        S[1 * 64] = ux.y;
-        S[2 * 64] = uy.x;
-        S[3 * 64] = uy.y;
-        S[4 * 64] = (double)ue;
-        const double2 t00 = make_double2(T[0], T[64]), t01 = make_double2(T[128], T[192]);
-        const double2 t10 = make_double2(T[256], T[320]), t11 = make_double2(T[384], T[448]);
-        const int tE = (int)T[512];
-        const double2 nx = cadd(cmul(t00, ux), cmul(t01, uy));
-        const double2 ny = cadd(cmul(t10, ux), cmul(t11, uy));
-        ux = nx;
-        uy = ny;
-        ue += tE;
-        renorm(ux, uy, ue);
-    }
-    const double2 Dm = csub(cscale(tau, ux), cmul(SR, uy));  // D = Dm 2^ue
-    const int eN = ue;
-
-    // backward: v states at segment ends, (v_{hi-1}, v_hi)
+    for (int k = 0; k < w; ++k) mat2_apply(get(k), ux, uy, ue);
     double2 vx = cscale(1.0 / tau, SR);  // v_N
     double2 vy = make_double2(1.0, 0.0);  // v_{N-1}
     int ve = 0;
-    for (int s = P - 1; s >= 0; --s) {
-        const double* T = lp.Tm + (item0 + s) * 9 * 64 + lane;
-        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
-        S[5 * 64] = vy.x;
-        S[6 * 64] = vy.y;
-        S[7 * 64] = vx.x;
-        S[8 * 64] = vx.y;
-        S[9 * 64] = (double)ve;
-        const int lo = min(N, s * L);
-        const int hi = min(N, lo + L);
-        if (lo >= hi) continue;  // empty tail segment: identity
-        const double2 t00 = make_double2(T[0], T[64]), t01 = make_double2(T[128], T[192]);
-        const double2 t10 = make_double2(T[256], T[320]), t11 = make_double2(T[384], T[448]);
-        const int tE = (int)T[512];
-        // det(T_s) = t_{lo-1} / t_{hi-1}; t_{lo-1} is odd (lo even)
-        const double t_hm1 = ((hi - 1) & 1) ? to : te;
-        const double idet = t_hm1 / to;
-        // (v_lo, v_{lo-1}) = adj(T) (v_hi, v_{hi-1}) 2^tE / det
-        const double2 nx = cscale(idet, csub(cmul(t11, vx), cmul(t01, vy)));
-        const double2 ny = cscale(idet, csub(cmul(t00, vy), cmul(t10, vx)));
-        vx = nx;
-        vy = ny;
-        ve += tE;
-        renorm(vx, vy, ve);
+    for (int k = SCAN_WAVES - 1; k > w; --k) {
+        const int k0 = range_lo(k), k1 = range_lo(k + 1);
+        if (k0 < k1) mat2_apply_inv(get(k), inv_det(k0, k1), vx, vy, ve);
     }
-    if (valid) {
+
+    // phase C: per-segment states through this wave's range
+    for (int c = s0; c < s1; c += 4) {
+        Mat2 Tc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < s1) Tc[q] = cached ? Tk[q] : load_T(lp, item0 + c + q, lane);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < s1) {
+                double* S = lp.St + (item0 + c + q) * 10 * 64 + lane;
+                S[0 * 64] = ux.x;
+                S[1 * 64] = ux.y;
+                S[2 * 64] = uy.x;
+                S[3 * 64] = uy.y;
+                S[4 * 64] = (double)ue;
+                mat2_apply(Tc[q], ux, uy, ue);
+            }
+    }
+    if (w == SCAN_WAVES - 1 && valid) {
+        // this wave's u now sits at the end of the chain: (u_N, u_{N-1})
+        const double2 Dm = csub(cscale(tau, ux), cmul(SR, uy));  // D = Dm 2^ue
+        const int eN = ue;
         double* Dp = lp.Dd + (long long)chain * 3;
         Dp[0] = Dm.x;
         Dp[1] = Dm.y;
@@ -366,6 +460,32 @@ __global__ __launch_bounds__(256) void scan_kernel(WLaunch lp, int ngroups)
             double gN = __builtin_amdgcn_ldexp(1.0, -eN);
             double GamL = -2.0 * SL.y, GamR = -2.0 * SR.y;
             lp.chain_out[(long long)ch.out_slot * 2 + 1] = GamL * GamR * iD2 * gN * gN;
+        }
+    }
+    auto v_step = [&](int s, const Mat2& Ts) {
+        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
+        S[5 * 64] = vy.x;  // (v_{hi-1}, v_hi)
+        S[6 * 64] = vy.y;
+        S[7 * 64] = vx.x;
+        S[8 * 64] = vx.y;
+        S[9 * 64] = (double)ve;
+        const int lo = min(N, s * L);
+        const int hi = min(N, lo + L);
+        if (lo < hi) mat2_apply_inv(Ts, inv_det(s, s + 1), vx, vy, ve);
+    };
+    if (cached) {
+#pragma unroll
+        for (int q = 3; q >= 0; --q)
+            if (s0 + q < s1) v_step(s0 + q, Tk[q]);
+    } else {
+        for (int c = s1 - 1; c >= s0; c -= 4) {
+            Mat2 Tc[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c - q >= s0) Tc[q] = load_T(lp, item0 + c - q, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c - q >= s0) v_step(c - q, Tc[q]);
         }
     }
 }
@@ -608,7 +728,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 // exact power of two of the next v checkpoint, so u needs no renormalisation:
 // |ut_n v_n| = |coef G_nn| stays bounded.  Checkpoint indices are static.
 template <int K, int NS>
-__global__ __launch_bounds__(256, 2) void sweep_eq_full_kernel(WLaunch lp)
+__global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLaunch lp)
 {
     __shared__ double red_all[4][K * 65];
     __shared__ double us_all[4][K * NS];
@@ -749,6 +869,37 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restri
     double acc = 0.0;
     for (int r = r_begin; r < r_end; ++r) acc += p[(long long)r * N];
     dst[dst_off[t] + (long long)c * N + n] = acc;
+}
+
+// One-pass fixed-order sum of the group rows of every tube: a block covers
+// 16 sites x 16 row slices (slice k sums rows k, k+16, ...), the slices are
+// then added in slice order.
+__global__ __launch_bounds__(256) void reduce_rows1_kernel(const double* __restrict__ src,
+                                                           const long long* __restrict__ src_off,
+                                                           const int* __restrict__ nrows,
+                                                           const int* __restrict__ Ns, double* __restrict__ dst,
+                                                           const long long* __restrict__ dst_off)
+{
+    __shared__ double red[16][17];
+    const int t = blockIdx.y;
+    const int N = Ns[t];
+    const int sl = threadIdx.x & 15;   // site within the block
+    const int k = threadIdx.x >> 4;    // row slice
+    const int n = blockIdx.x * 16 + sl;
+    const int R = nrows[t];
+    double acc = 0.0;
+    if (n < N) {
+        const double* p = src + src_off[t] + n;
+        for (int r = k; r < R; r += 16) acc += p[(long long)r * N];
+    }
+    red[k][sl] = acc;
+    __syncthreads();
+    if (k == 0 && n < N) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) s += red[q][sl];
+        dst[dst_off[t] + n] = s;
+    }
 }
 
 // Per-chain DOS trace: sum the per-segment partials in a fixed order.

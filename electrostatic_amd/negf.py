@@ -154,6 +154,67 @@ class Tube:
         return d, t
 
 
+class Batch:
+    """Many tubes solved in one launch sequence (negf_batch_*)."""
+
+    def __init__(self, tubes):
+        self.lib = _lib.load()
+        self.tubes = list(tubes)
+        arr = (C.c_void_p * len(self.tubes))(*[t.h for t in self.tubes])
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_batch_create(arr, len(self.tubes), C.byref(h)))
+        self.h = h
+        off = (C.c_longlong * (len(self.tubes) + 1))()
+        _lib.check(self.lib.negf_batch_offsets(h, off))
+        self.offsets = np.array(off[:], dtype=np.int64)
+
+    def close(self):
+        if self.h:
+            self.lib.negf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve_device(self, iter, dQ_ptr, stream=None):
+        _lib.check(self.lib.negf_batch_solve(self.h, iter, C.c_void_p(dQ_ptr),
+                                             C.c_void_p(stream) if stream else None))
+
+    def use_graph(self, on=True):
+        _lib.check(self.lib.negf_batch_use_graph(self.h, int(on)))
+
+    def set_segment_length(self, L):
+        _lib.check(self.lib.negf_batch_set_segment_length(self.h, L))
+
+    def autotune(self, dQ_ptr):
+        L = C.c_int()
+        ms = C.c_double()
+        _lib.check(self.lib.negf_batch_autotune(self.h, C.c_void_p(dQ_ptr), C.byref(L), C.byref(ms)))
+        return L.value, ms.value
+
+    def enable_timing(self, on=True):
+        _lib.check(self.lib.negf_batch_enable_timing(self.h, int(on)))
+
+    def last_time_ms(self):
+        ms = C.c_double()
+        _lib.check(self.lib.negf_batch_last_time(self.h, C.byref(ms)))
+        return ms.value
+
+    def energy_sites(self):
+        v = C.c_double()
+        _lib.check(self.lib.negf_batch_energy_sites(self.h, C.byref(v)))
+        return v.value
+
+    def status(self):
+        rc = self.lib.negf_batch_status(self.h)
+        if rc not in (0, 3):
+            _lib.check(rc)
+        return rc
+
+
 class Broyden:
     """Broyden modified-second mixer on the GPU (Broyden_Second_Parallel.cpp)."""
 

@@ -56,6 +56,9 @@ typedef struct {
     double Uc[2];             /* contact potentials (U_contact)               */
     int use_decimation;       /* 0: closed-form lead GF, 1: decimation        */
     int decimation_layers, decimation_max_iter, decimation_rel_error;
+    int segment_length;       /* RGF segment length in sites, a multiple of  */
+                              /* 16 up to 128 (0: default 128); performance   */
+                              /* only, results do not depend on it           */
 } negf_chain_params;
 
 /* acc[n] = sum_k Re(coef_k * G_nn(E_k)) for n < N (overwritten).
@@ -141,6 +144,8 @@ typedef struct {
     int integrand_correction_interval;
     double kT_window_around_singularity[2];
     int flatband_dos_integration_pts;
+    int segment_length;             /* RGF segment length (0: default); no */
+                                    /* reference counterpart (performance)  */
 } negf_tube_params;
 
 /* Fill p with the reference defaults (NEGF_Common.H:384-476, CNT.H:17-18). */
@@ -205,6 +210,38 @@ int negf_tube_current(negf_tube *t, double *I);
  * T = sum_modes degen * Gamma_L Gamma_R |G_0,N-1|^2 (host arrays). */
 int negf_tube_dos(negf_tube *t, int nE, const double *E_real, double *dos,
                   double *trans);
+
+/* ------------------------------------------------------------------------
+ * Batched solve of many tubes (no reference counterpart: the reference loops
+ * over tubes, each with its own per-energy host loop, Transport.cpp:417-431).
+ * All tubes' equilibrium chains form one RGF launch sequence and all biased
+ * tubes' non-equilibrium chains a second one; Q of tube i lands at
+ * d_Q[offsets[i] .. offsets[i+1]).  Tubes keep their own parameters,
+ * contacts and potentials (negf_tube_set_* as usual) and must share the lead
+ * (decimation) settings.
+ * ------------------------------------------------------------------------ */
+typedef struct negf_batch negf_batch;
+int negf_batch_create(negf_tube *const *tubes, int ntubes, negf_batch **out);
+int negf_batch_destroy(negf_batch *b);
+/* offsets: ntubes + 1 values (cumulative site counts). */
+int negf_batch_offsets(const negf_batch *b, long long *offsets);
+/* Solve_NEGF for every tube at SCF iteration iter; d_Q is ready on stream. */
+int negf_batch_solve(negf_batch *b, int iter, double *d_Q, void *stream);
+/* Replay the launch sequence from a captured hipGraph (re-captured when a
+ * tube's paths change or d_Q moves). */
+int negf_batch_use_graph(negf_batch *b, int on);
+/* RGF segment length for the whole batch (0: default). */
+int negf_batch_set_segment_length(negf_batch *b, int segment_length);
+/* Time the solve for segment lengths 32..128 and keep the fastest. */
+int negf_batch_autotune(negf_batch *b, double *d_Q, int *segment_length,
+                        double *ms);
+/* hipEvent time of the last solve's launch sequence. */
+int negf_batch_enable_timing(negf_batch *b, int on);
+int negf_batch_last_time(negf_batch *b, double *ms);
+/* Energy-site count of one batched solve (chains x sites). */
+int negf_batch_energy_sites(const negf_batch *b, double *energy_sites);
+/* Synchronise; NEGF_EROOT (diagnostic) as for negf_tube_status. */
+int negf_batch_status(negf_batch *b);
 
 /* ------------------------------------------------------------------------
  * Broyden modified-second on device (Execute_Broyden_Modified_Second_

@@ -30,7 +30,7 @@ def to_np(t, complex_=False):
 
 
 def chain_params(N, t_even, t_odd, Uc, use_decimation=0, layers=10, max_iter=30,
-                 rel_error=0):
+                 rel_error=0, segment_length=0):
     p = _lib.negf_chain_params()
     p.N = N
     p.t_even = t_even
@@ -40,6 +40,7 @@ def chain_params(N, t_even, t_odd, Uc, use_decimation=0, layers=10, max_iter=30,
     p.decimation_layers = layers
     p.decimation_max_iter = max_iter
     p.decimation_rel_error = rel_error
+    p.segment_length = segment_length
     return p
 
 

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_version_and_limits():
     lib = _lib.load()
     assert lib.negf_version() >= 100
-    assert lib.negf_max_sites() == 8192
+    assert lib.negf_max_sites() == 8192 * 128
 
 
 def test_tube_param_defaults_follow_the_reference():

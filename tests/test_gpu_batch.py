@@ -1,0 +1,116 @@
+"""GPU tests of the batched multi-tube solve (negf_batch_*): per-tube charge
+equals each tube's own solve and the oracle; graph replay and the segment
+length autotuner do not change results.
+
+Tolerance: 1e-10 relative to the oracle; 1e-12 between launch forms (the
+batch groups chains differently, so partial sums round differently).
+"""
+import numpy as np
+import pytest
+import torch
+
+from electrostatic_amd import negf
+from electrostatic_amd.workloads import gaussian_bumps
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def make_tubes(cells, biases, modes=2, seed=0):
+    tubes, Us = [], []
+    for i, (nc, V) in enumerate(zip(cells, biases)):
+        t = negf.Tube(num_unitcells=nc, num_modes=modes, contact_mu=(-1.0, -1.0 - V),
+                      eq_integration_pts=(24, 24, 32), noneq_integration_pts=(48,),
+                      E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0)
+        U = gaussian_bumps(t.N, seed=seed + i, amp=0.25)
+        t.set_potential(U)
+        tubes.append(t)
+        Us.append(U)
+    return tubes, Us
+
+
+def test_batch_matches_single_tube_solves_and_oracle(dev):
+    cells = [10, 37, 3, 120, 64]
+    biases = [0.0, 0.2, 0.35, 0.0, 0.1]
+    tubes, Us = make_tubes(cells, biases)
+    singles = [t.solve(1)[0] for t in tubes]
+    batch = negf.Batch(tubes)
+    dQ = torch.zeros(int(batch.offsets[-1]), dtype=torch.float64, device=dev)
+    batch.solve_device(1, dQ.data_ptr())
+    torch.cuda.synchronize()
+    Q = dQ.cpu().numpy()
+    for i, t in enumerate(tubes):
+        q = Q[batch.offsets[i]:batch.offsets[i + 1]]
+        assert rel(q, singles[i]) < 1e-12, i
+    # one tube against the oracle
+    i = 1
+    _, betas, degs = tubes[i].modes()
+    ot = O.make_tube(tubes[i].N, list(betas), list(degs), mu=(-1.0, -1.2),
+                     eq_pts=(24, 24, 32), noneq_pts=(48,), eta=1e-5,
+                     E_valence_min=-10.0, E_pole_max=3.0)
+    r0, _ = O.rho0(ot)
+    Qo, _, _, _ = O.solve_negf(ot, Us[i], (Us[i][0], Us[i][-1]), r0)
+    assert rel(Q[batch.offsets[i]:batch.offsets[i + 1]], Qo) < 1e-10
+    batch.close()
+
+
+def test_batch_graph_replay_and_autotune(dev):
+    tubes, Us = make_tubes([30, 7, 80], [0.15, 0.0, 0.3], modes=1, seed=5)
+    batch = negf.Batch(tubes)
+    n = int(batch.offsets[-1])
+    d1 = torch.zeros(n, dtype=torch.float64, device=dev)
+    batch.solve_device(1, d1.data_ptr())
+    torch.cuda.synchronize()
+    ref = d1.cpu().numpy()
+    batch.use_graph(True)
+    d2 = torch.empty(n, dtype=torch.float64, device=dev)
+    for rep in range(3):
+        d2.fill_(float("nan"))            # the replay must rewrite every site
+        batch.solve_device(2 + rep, d2.data_ptr())
+        torch.cuda.synchronize()
+        assert rel(d2.cpu().numpy(), ref) < 1e-14
+        # eager work between replays
+        d1.add_(1.0)
+        torch.cuda.synchronize()
+    L, ms = batch.autotune(d1.data_ptr())
+    assert L in (32, 48, 64, 96, 128) and ms > 0
+    d3 = torch.zeros(n, dtype=torch.float64, device=dev)
+    batch.solve_device(5, d3.data_ptr())
+    torch.cuda.synchronize()
+    assert rel(d3.cpu().numpy(), ref) < 1e-12
+    # a contact change in one tube rebuilds the batch (and its graph)
+    tubes[1].set_contacts((-1.0, -1.25))
+    batch.solve_device(6, d3.data_ptr())
+    torch.cuda.synchronize()
+    q1, _ = tubes[1].solve(7)
+    assert rel(d3.cpu().numpy()[batch.offsets[1]:batch.offsets[2]], q1) < 1e-12
+    batch.close()
+
+
+def test_batch_long_uneven_tubes(dev):
+    # lengths far beyond one segment scan of 64 segments, up to 20000 sites
+    # (the 20000-site tube is in equilibrium: the oracle's O(N^2) contact walk
+    # would take minutes at that length)
+    tubes, Us = make_tubes([125, 5000, 1210], [0.3, 0.0, 0.1], modes=1, seed=9)
+    batch = negf.Batch(tubes)
+    dQ = torch.zeros(int(batch.offsets[-1]), dtype=torch.float64, device=dev)
+    batch.solve_device(1, dQ.data_ptr())
+    torch.cuda.synchronize()
+    Q = dQ.cpu().numpy()
+    i = 1   # N = 20000
+    _, betas, degs = tubes[i].modes()
+    ot = O.make_tube(tubes[i].N, list(betas), list(degs), mu=(-1.0, -1.0),
+                     eq_pts=(24, 24, 32), noneq_pts=(48,), eta=1e-5,
+                     E_valence_min=-10.0, E_pole_max=3.0)
+    r0, _ = O.rho0(ot)
+    Qo, _, _, _ = O.solve_negf(ot, Us[i], (Us[i][0], Us[i][-1]), r0)
+    assert rel(Q[batch.offsets[i]:batch.offsets[i + 1]], Qo) < 1e-10
+    # the biased long tube against its own single-tube solve
+    for j in (0, 2):
+        qj, _ = tubes[j].solve(1)
+        assert rel(Q[batch.offsets[j]:batch.offsets[j + 1]], qj) < 1e-12
+    batch.close()

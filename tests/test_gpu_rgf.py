@@ -78,6 +78,33 @@ def test_diag_config2_shape():
     assert rel_err(acc, acc_ref) < 1e-10
 
 
+@pytest.mark.parametrize("N,L", [(64, 0), (257, 0), (1000, 16), (1000, 48), (1000, 128),
+                                 (4096, 96), (20011, 0), (20011, 128)])
+def test_charge_only_path_matches_oracle(N, L):
+    """Charge without diag G^R: full segments take the unrolled sweep, the
+    last (partial) one the general sweep; chains far beyond 8192 sites."""
+    U = smooth_potential(N, seed=N + L)
+    E = energies(48, 48, seed=N)
+    rng = np.random.default_rng(N)
+    coef = rng.normal(size=E.size) + 1j * rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    acc_ref, st = O.accumulate_diagG(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E, coef)
+    acc, _ = run_diag(U, BETA17, GAMMA, Uc, E, coef, want_G=False, segment_length=L)
+    assert rel_err(acc, acc_ref) < 1e-10
+
+
+def test_segment_length_does_not_change_results():
+    N = 1234
+    U = smooth_potential(N, seed=7)
+    E = energies(70, 70, seed=7)
+    coef = np.exp(1j * np.linspace(0, 2, E.size))
+    Uc = (U[0], U[-1])
+    outs = [run_diag(U, BETA17, GAMMA, Uc, E, coef, want_G=False, segment_length=L)[0]
+            for L in (16, 32, 64, 80, 128)]
+    for o in outs[1:]:
+        assert rel_err(o, outs[0]) < 1e-12
+
+
 def test_diag_decimation_matches_oracle():
     N = 64
     U = smooth_potential(N, seed=5)
