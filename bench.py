@@ -110,6 +110,53 @@ def scf_sweep(with_cpu=True):
     return out
 
 
+def batch_leg(dev, steps=10):
+    """64 (17,0) tubes, lengths log-uniform in [500, 20000] sites (seed 3),
+    2 modes each, 256 energies per tube (contour 32/32/64 + 128 real-axis
+    points for the tube's own bias V ~ U(0, 0.5) V, contacts at -1 and -1-V eV),
+    smooth potentials; one batched solve per step (all tubes in one launch
+    sequence), eager and replayed from a captured hipGraph, segment length
+    from the autotuner."""
+    import torch
+    from electrostatic_amd import negf
+    from electrostatic_amd.workloads import smooth_potential, tube_batch
+    Ns, biases = tube_batch(n_tubes=64, lo=500, hi=20000, seed=3, bias_max=0.5)
+    tubes = []
+    for i, (N, V) in enumerate(zip(Ns, biases)):
+        cells = max(1, int(round(N / 4)))
+        t = negf.Tube(num_unitcells=cells, num_modes=2, contact_mu=(-1.0, -1.0 - V),
+                      eq_integration_pts=(32, 32, 64), noneq_integration_pts=(128,),
+                      E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0)
+        t.set_potential(smooth_potential(t.N, seed=100 + i, lo=-0.3, hi=0.3))
+        tubes.append(t)
+    batch = negf.Batch(tubes)
+    n = int(batch.offsets[-1])
+    dQ = torch.zeros(n, dtype=torch.float64, device=dev)
+    L, tune_ms = batch.autotune(dQ.data_ptr())
+    es = batch.energy_sites()
+    out = {"workload": "64 uneven (17,0) tubes, N log-uniform [500, 20000] (seed 3), 2 modes, "
+                       "256 energies per tube (32/32/64 contour + 128 real axis), per-tube bias U(0, 0.5) V",
+           "sites": n, "energy_sites": es, "segment_length": L}
+    for graph in (False, True):
+        batch.use_graph(graph)
+        batch.solve_device(1, dQ.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            batch.solve_device(1, dQ.data_ptr())
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        key = "graph" if graph else "eager"
+        out[f"ms_per_solve_{key}"] = ms
+        out[f"energy_site_per_s_{key}"] = es / (ms * 1e-3)
+    if not np.all(np.isfinite(dQ.cpu().numpy())):
+        raise RuntimeError("non-finite batched charge")
+    batch.close()
+    for t in tubes:
+        t.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--segment-length", type=int, default=0,
                     help="RGF segment length (0: library default)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the secondary 64-tube batched-solve measurement")
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the secondary SCF gate-sweep measurement")
     ap.add_argument("--write-diag-g", action="store_true",
@@ -240,6 +289,8 @@ def main():
     lib.negf_rgf_plan_destroy(plan)
     if not args.no_scf:
         out["scf"] = scf_sweep(with_cpu=not args.no_cpu_baseline)
+    if not args.no_batch:
+        out["batch"] = batch_leg(dev)
     print(json.dumps(out), flush=True)
 
 

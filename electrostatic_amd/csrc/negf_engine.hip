@@ -93,6 +93,21 @@ static void launch_sweep(int nblk, const WLaunch& lp, hipStream_t st)
     hipLaunchKernelGGL((sweep_kernel<NEGF_SUB_K, NEGF_SUB_NS, KIND, WG>), dim3(nblk), dim3(256), 0, st, lp);
 }
 
+static void launch_sweep_neq_full(int ns, int nblk, const WLaunch& lp, hipStream_t st)
+{
+    constexpr int K = NEGF_SUB_K;
+    switch (ns) {
+        case 1: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 1>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 2: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 2>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 3: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 3>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 4: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 4>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 5: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 5>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 6: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 6>), dim3(nblk), dim3(256), 0, st, lp); break;
+        case 7: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 7>), dim3(nblk), dim3(256), 0, st, lp); break;
+        default: hipLaunchKernelGGL((sweep_neq_full_kernel<K, 8>), dim3(nblk), dim3(256), 0, st, lp); break;
+    }
+}
+
 static void launch_sweep_eq_full(int ns, int nblk, const WLaunch& lp, hipStream_t st)
 {
     constexpr int K = NEGF_SUB_K;
@@ -206,7 +221,8 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
             // full segments of the equilibrium charge go to the unrolled sweep
             const int sgi = i % P;
             const bool full = sgi * L + L <= tb.N;
-            if (kind == KIND_EQ && !write_G && full)
+            const bool samples = tb.sample[0] >= 0 || tb.sample[1] >= 0 || tb.sample[2] >= 0;
+            if (full && ((kind == KIND_EQ && !write_G) || (kind == KIND_NONEQ && !samples)))
                 fast_items[L / NEGF_SUB_K].push_back(items + i);
             else
                 slow_items.push_back(items + i);
@@ -328,7 +344,10 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         WLaunch lf = lp;
         lf.item_list = d_item_list_.as<int>() + fast_off_[ns];
         lf.nlist = fast_n_[ns];
-        launch_sweep_eq_full(ns, (fast_n_[ns] + 3) / 4, lf, st);
+        if (kind_ == KIND_EQ)
+            launch_sweep_eq_full(ns, (fast_n_[ns] + 3) / 4, lf, st);
+        else
+            launch_sweep_neq_full(ns, (fast_n_[ns] + 3) / 4, lf, st);
     }
     lp.item_list = d_item_list_.as<int>() + slow_off_;
     lp.nlist = slow_n_;

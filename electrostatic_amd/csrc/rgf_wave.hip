@@ -817,6 +817,98 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     }
 }
 
+// Non-equilibrium sweep of a full segment (no integrand samples): the
+// contribution of site n is kL Gamma_L |G_n0|^2 + kR Gamma_R |G_n,N-1|^2 with
+// G_n0 = v_n / D and G_n,N-1 = u_n / D.  The weights are folded into the
+// states: ut = sqrt|kR Gamma_R| 2^(e_u - e_D) u / |D| (a solution of the same
+// recurrence, never renormalised: |ut_n| is |G_n,N-1| up to the constant) and,
+// per sub-block, vt = sqrt|kL Gamma_L| 2^(e_v - e_D) v / |D|; the signs of the
+// two weights multiply the squared moduli.
+template <int K, int NS>
+__global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_neq_full_kernel(WLaunch lp)
+{
+    __shared__ double red_all[4][K * 65];
+    __shared__ double us_all[4][K * NS];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
+    if (idx >= lp.nlist) return;
+    const int item = __builtin_amdgcn_readfirstlane(lp.item_list[idx]);
+    double* red = red_all[wv];
+    WTube tube;
+    ItemCtx ic = item_ctx(lp, item, lane, tube);
+    const ChainDesc ch = lp.chains[ic.chain];
+    const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
+    const int N = tube.N;
+    const int lo = ic.s * tube.L;
+    double* us = us_all[wv];
+    stage_segment(us, tube.U, lo, lo + K * NS, lane);
+    const double* S = lp.St + (long long)item * 10 * 64 + lane;
+    double2 ux = make_double2(S[0], S[64]), uy = make_double2(S[128], S[192]);
+    const int ue = (int)S[256];
+    double2 x = make_double2(S[320], S[384]), y = make_double2(S[448], S[512]);
+    int e = (int)S[576];
+    const double* Dp = lp.Dd + (long long)ic.chain * 3;
+    const double iD2 = 1.0 / cabs2(make_double2(Dp[0], Dp[1]));
+    const int eN = (int)Dp[2];
+    const double vmask = ic.valid ? 1.0 : 0.0;
+    const double wL = ch.cL * vmask * (-2.0 * ch.SLi) * iD2;  // kL Gamma_L / |Dm|^2
+    const double wR = ch.cR * vmask * (-2.0 * ch.SRi) * iD2;
+    const double sgnL = wL < 0.0 ? -1.0 : 1.0, sgnR = wR < 0.0 ? -1.0 : 1.0;
+    const double rL = sqrt(fabs(wL)), rR = sqrt(fabs(wR));
+    double* prow = lp.part + tube.part_off + (long long)ic.g * N + lo;
+
+    double2 cx[NS], cy[NS];
+    int ce[NS];
+    cx[NS - 1] = x;
+    cy[NS - 1] = y;
+    ce[NS - 1] = e;
+#pragma unroll
+    for (int j = NS - 1; j >= 1; --j) {
+#pragma unroll
+        for (int i = K - 1; i >= 1; i -= 2) {
+            bwd_step<1>(k, us[j * K + i], x, y);
+            bwd_step<0>(k, us[j * K + i - 1], x, y);
+        }
+        renorm(x, y, e);
+        cx[j - 1] = x;
+        cy[j - 1] = y;
+        ce[j - 1] = e;
+    }
+    {
+        const double su = rR * __builtin_amdgcn_ldexp(1.0, ue - eN);
+        ux = cscale(su, ux);
+        uy = cscale(su, uy);
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double2 vr[K];
+        {
+            const double sv = rL * __builtin_amdgcn_ldexp(1.0, ce[j] - eN);
+            double2 xx = cscale(sv, cx[j]), yy = cscale(sv, cy[j]);
+            vr[K - 1] = xx;
+#pragma unroll
+            for (int i = K - 1; i >= 1; --i) {
+                if (i & 1)
+                    bwd_step<1>(k, us[j * K + i], xx, yy);
+                else
+                    bwd_step<0>(k, us[j * K + i], xx, yy);
+                vr[i - 1] = xx;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            red[i * 65 + lane] = fma(sgnL, cabs2(vr[i]), sgnR * cabs2(ux));
+            if (i & 1)
+                fwd_step<1>(k, us[j * K + i], ux, uy);
+            else
+                fwd_step<0>(k, us[j * K + i], ux, uy);
+        }
+        const double r = lane_sum_transpose<K>(red, lane);
+        if (lane < K) prow[j * K + lane] = r;
+    }
+}
+
 // Lead self-energies of every chain, Sigma = tau g tau^+ with tau = t_odd
 // (get_Sigma_at_contacts NEGF_Common.cpp:4165-4177, CNT.cpp:348-351); one
 // thread per chain.

@@ -164,6 +164,37 @@ def test_noneq_accumulate_matches_oracle_walk(N):
             assert abs(got[k, w] - r[n]) <= 1e-10 * max(abs(r[n]), 1e-300) + 1e-300
 
 
+@pytest.mark.parametrize("N,L", [(128, 0), (1000, 0), (1000, 64), (4096, 128), (20011, 0)])
+def test_noneq_without_samples_matches_oracle(N, L):
+    """Without integrand samples full segments take the unrolled
+    non-equilibrium sweep (signed weights folded into the states)."""
+    lib = _lib.load()
+    U = smooth_potential(N, seed=N + 3, lo=-0.3, hi=0.2)
+    E = np.concatenate([np.linspace(-2.5, 1.5, 50), [0.05, 0.1]]) + 1e-5j  # in-gap too
+    rng = np.random.default_rng(N + 1)
+    cL = rng.normal(size=E.size)
+    cR = rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    p = chain_params(N, BETA17, GAMMA, Uc, segment_length=L)
+    dU, dE = dev_array(U), dev_array(E)
+    dL, dR = dev_array(cL), dev_array(cR)
+    acc = dev_zeros(N)
+    _lib.check(lib.negf_rgf_noneq_accumulate(p, ptr(dU), E.size, ptr(dE), ptr(dL),
+                                             ptr(dR), None, None, ptr(acc), None, None))
+    if N <= 4096:
+        ref, st = O.accumulate_noneq(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E, cL, cR)
+        assert rel_err(to_np(acc), ref) < 1e-10
+    else:
+        # the oracle's O(N^2) walk is too slow at this length: compare with the
+        # general sweep (integrand samples requested)
+        acc2 = dev_zeros(N)
+        integ = dev_zeros(3 * E.size)
+        _lib.check(lib.negf_rgf_noneq_accumulate(p, ptr(dU), E.size, ptr(dE), ptr(dL),
+                                                 ptr(dR), None, None, ptr(acc2),
+                                                 ptr(integ), None))
+        assert rel_err(to_np(acc), to_np(acc2)) < 1e-12
+
+
 def test_dos_transmission_matches_oracle():
     lib = _lib.load()
     N = 120
