@@ -122,6 +122,13 @@ SIGNATURES = {
     "negf_batch_last_time": (C.c_int, [vp, dp]),
     "negf_batch_energy_sites": (C.c_int, [vp, dp]),
     "negf_batch_status": (C.c_int, [vp]),
+    "negf_batch_scf_create": (C.c_int, [vp, C.POINTER(vp), C.c_int, C.POINTER(negf_scf_params), C.POINTER(vp)]),
+    "negf_batch_scf_destroy": (C.c_int, [vp]),
+    "negf_batch_scf_set_external_potential": (C.c_int, [vp, dp]),
+    "negf_batch_scf_set_input_charge": (C.c_int, [vp, dp]),
+    "negf_batch_scf_get_input_charge": (C.c_int, [vp, dp]),
+    "negf_batch_scf_use_graph": (C.c_int, [vp, C.c_int]),
+    "negf_batch_scf_run": (C.c_int, [vp, dp, dp, dp, ip, dp, ip]),
     "negf_broyden_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]),
     "negf_broyden_destroy": (C.c_int, [vp]),
     "negf_broyden_set_nin": (C.c_int, [vp, dp]),

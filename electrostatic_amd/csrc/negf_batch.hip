@@ -124,15 +124,29 @@ class Batch {
         return NEGF_OK;
     }
 
-    int enqueue(double* dQ)
+    int enqueue(double* dQ) { return enqueue_on(dQ, stream); }
+    int enqueue_on(double* dQ, hipStream_t st)
     {
         int rc;
-        if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, stream))) return rc;
-        if (n_neq > 0 && (rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, stream))) return rc;
+        if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st))) return rc;
+        if (n_neq > 0 && (rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, st))) return rc;
         const long long Nt = off.back();
-        hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, stream, Nt,
+        hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, st, Nt,
                            d_eq.as<double>(), d_neq.as<double>(), d_rho0.as<double>(), dQ);
         NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
+    // host part of a solve for every tube; rebuild the plans if any paths changed
+    int prepare(int iter, bool* rebuilt)
+    {
+        int rc;
+        bool changed = false;
+        for (size_t i = 0; i < tubes.size(); ++i) {
+            if ((rc = tubes[i]->prepare(iter))) return rc;
+            changed |= tubes[i]->paths_version != versions[i];
+        }
+        if (changed && (rc = build())) return rc;
+        *rebuilt = changed;
         return NEGF_OK;
     }
 
@@ -283,6 +297,11 @@ using namespace negf;
 struct negf_batch {
     Batch b;
 };
+
+namespace negf {
+int batch_prepare(negf_batch* b, int iter, bool* rebuilt) { return b->b.prepare(iter, rebuilt); }
+int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st) { return b->b.enqueue_on(dQ, st); }
+}  // namespace negf
 
 extern "C" {
 

@@ -117,6 +117,12 @@ void gl_path(double ar, double ai, double br, double bi, int n, int type, double
 // Segmentation of a chain of N sites into P segments of L sites (L a
 // multiple of NEGF_SUB_K, at most NEGF_SEG_MAX); 0 if N is too long.
 int choose_segments(int N, int L_req, int* P, int* L);
+
+// Batched solve internals shared with the batched SCF: the host part of a
+// solve (rebuilding plans when paths changed) and the bare launch sequence
+// on a given stream (capturable).
+int batch_prepare(negf_batch* b, int iter, bool* rebuilt);
+int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st);
 int max_sites();
 
 }  // namespace negf

@@ -65,9 +65,9 @@ __device__ void history_dots(const BroydenDev& b, int S, int m_hi, const double*
     }
 }
 
-__global__ __launch_bounds__(kBroydenThreads) void broyden_step_kernel(int S, int m, double fraction, int norm_type,
-                                                                         const double* __restrict__ n_out,
-                                                                         BroydenDev b)
+// One Broyden update of one system by one workgroup of kBroydenThreads.
+__device__ void broyden_update(int S, int m, double fraction, int norm_type, const double* __restrict__ n_out,
+                               const BroydenDev& b)
 {
     __shared__ double red[3][kWaves];
     __shared__ double hist[512];
@@ -146,6 +146,13 @@ __global__ __launch_bounds__(kBroydenThreads) void broyden_step_kernel(int S, in
         b.n_prev[s] = prev;
         b.n_in[s] = prev - fraction * b.F[s] - sv;
     }
+}
+
+__global__ __launch_bounds__(kBroydenThreads) void broyden_step_kernel(int S, int m, double fraction, int norm_type,
+                                                                         const double* __restrict__ n_out,
+                                                                         BroydenDev b)
+{
+    broyden_update(S, m, fraction, norm_type, n_out, b);
 }
 
 // U_i = U_ext_i - V_gate - sum_j K_ij n_j: one wavefront per row, lanes
@@ -280,6 +287,318 @@ static void coulomb_kernel(const std::vector<double>& ptd, const negf_scf_params
                             1.0 / std::sqrt(d2 + p.R_image_nm * p.R_image_nm));
         }
 }
+
+// ---------------------------------------------------------------- batched SCF
+// Many equal tubes coupled to their own charge through one shared dense
+// kernel K (N x N): every iteration is a GEMM U_t = U_ext,t - V_t - K n_t for
+// all tubes, one batched NEGF solve, and one Broyden workgroup per tube.
+
+// Y[t][i] = sum_j K[i][j] X[t][j] (K row-major N x N, X and Y T x N):
+// 32 x 32 output tiles, 2 x 2 per thread, 16-wide reduction chunks in LDS.
+__global__ __launch_bounds__(256) void dgemm_kx_kernel(int N, int T, const double* __restrict__ K,
+                                                       const double* __restrict__ X, size_t ldx,
+                                                       double* __restrict__ Y)
+{
+    __shared__ double Ks[32][17];
+    __shared__ double Xs[32][17];
+    const int i0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads
+    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+    for (int j0 = 0; j0 < N; j0 += 16) {
+        // 32 x 16 tiles of K (rows i) and X (rows t)
+        for (int q = threadIdx.x; q < 512; q += 256) {
+            const int r = q >> 4, c = q & 15;
+            const int i = i0 + r, t = t0 + r, j = j0 + c;
+            Ks[r][c] = (i < N && j < N) ? K[(size_t)i * N + j] : 0.0;
+            Xs[r][c] = (t < T && j < N) ? X[(size_t)t * ldx + j] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const double k0 = Ks[ty][c], k1 = Ks[ty + 16][c];
+            const double x0 = Xs[tx][c], x1 = Xs[tx + 16][c];
+            acc[0][0] = fma(k0, x0, acc[0][0]);
+            acc[0][1] = fma(k0, x1, acc[0][1]);
+            acc[1][0] = fma(k1, x0, acc[1][0]);
+            acc[1][1] = fma(k1, x1, acc[1][1]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+            const int i = i0 + ty + 16 * a, t = t0 + tx + 16 * b;
+            if (i < N && t < T) Y[(size_t)t * N + i] = acc[a][b];
+        }
+}
+
+// U_t = U_ext,t - V_t - KN_t, scattered into each tube's potential and
+// contact potentials; also kept in Uall for the caller.
+__global__ void batch_potential_kernel(int N, int T, const double* __restrict__ Uext, const double* __restrict__ V,
+                                       const double* __restrict__ KN, double* const* __restrict__ dU,
+                                       double* const* __restrict__ dUc, double* __restrict__ Uall)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)N * T) return;
+    const int t = (int)(q / N), i = (int)(q - (long long)t * N);
+    const double u = (Uext ? Uext[q] : 0.0) - V[t] - KN[q];
+    Uall[q] = u;
+    dU[t][i] = u;
+    if (i == 0) dUc[t][0] = u;
+    if (i == N - 1) dUc[t][1] = u;
+}
+
+// One Broyden update per active tube (workgroup t); step counters on the
+// device so that the launch is identical every iteration.
+__global__ __launch_bounds__(kBroydenThreads) void broyden_batch_kernel(int S, int max_step, double fraction,
+                                                                          int norm_type, const double* __restrict__ n_out,
+                                                                          BroydenDev base, size_t state_stride,
+                                                                          size_t hist_stride, int* __restrict__ steps,
+                                                                          const int* __restrict__ active)
+{
+    const int t = blockIdx.x;
+    if (!active[t]) return;
+    const int m = steps[t] - 1;
+    BroydenDev b;
+    b.n_in = base.n_in + t * state_stride;
+    b.n_prev = base.n_prev + t * state_stride;
+    b.F = base.F + t * state_stride;
+    b.dF = base.dF + t * state_stride;
+    b.Norm = base.Norm + t * state_stride;
+    b.sum_vec = base.sum_vec + t * state_stride;
+    b.V = base.V + t * hist_stride;
+    b.W = base.W + t * hist_stride;
+    b.intermed = base.intermed + t * hist_stride;
+    b.out = base.out + (size_t)t * 4;
+    if (m > max_step) {
+        if (threadIdx.x == 0) b.out[3] = 1.0;  // step limit reached
+        return;
+    }
+    broyden_update(S, m, fraction, norm_type, n_out + (size_t)t * S, b);
+    if (threadIdx.x == 0) {
+        steps[t] = m + 2;
+        b.out[3] = 0.0;
+    }
+}
+
+// Keep the U and Q of tubes that converged at this iteration.
+__global__ void batch_keep_kernel(int N, int T, const int* __restrict__ keep, const double* __restrict__ U,
+                                  const double* __restrict__ Q, double* __restrict__ Uo, double* __restrict__ Qo)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)N * T) return;
+    const int t = (int)(q / N);
+    if (keep[t]) {
+        Uo[q] = U[q];
+        Qo[q] = Q[q];
+    }
+}
+
+class BatchScf {
+   public:
+    ~BatchScf()
+    {
+        drop_graph();
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    }
+    int init(negf_batch* b, negf_tube* const* tubes, int T_, const negf_scf_params& prm)
+    {
+        batch = b;
+        T = T_;
+        p = prm;
+        N = tubes[0]->t.N;
+        for (int t = 0; t < T; ++t) {
+            if (tubes[t]->t.N != N) {
+                set_error("batched SCF needs tubes of equal length (shared electrostatic kernel)");
+                return NEGF_EARG;
+            }
+            hU.push_back(tubes[t]->t.d_U.as<double>());
+            hUc.push_back(tubes[t]->t.d_Uc.as<double>());
+        }
+        coulomb_kernel(tubes[0]->t.ptd, p, K);
+        M = p.Broyden_threshold_maxstep;
+        state_stride = 6 * (size_t)N;
+        hist_stride = 2 * (size_t)(M + 1) * N + (M + 1);
+        int rc;
+        NEGF_HIP_TRY(hipStreamCreate(&st));
+        if ((rc = d_K.ensure(sizeof(double) * K.size()))) return rc;
+        if ((rc = d_Uext.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_V.ensure(sizeof(double) * T))) return rc;
+        if ((rc = d_KN.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_U.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_Q.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_Uo.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_Qo.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_ptrU.ensure(sizeof(double*) * T))) return rc;
+        if ((rc = d_ptrUc.ensure(sizeof(double*) * T))) return rc;
+        if ((rc = d_state.ensure(sizeof(double) * state_stride * T))) return rc;
+        if ((rc = d_hist.ensure(sizeof(double) * hist_stride * T))) return rc;
+        if ((rc = d_out.ensure(sizeof(double) * 4 * T))) return rc;
+        if ((rc = d_steps.ensure(sizeof(int) * T))) return rc;
+        if ((rc = d_active.ensure(sizeof(int) * T))) return rc;
+        if ((rc = d_keep.ensure(sizeof(int) * T))) return rc;
+        if ((rc = h_out.ensure(sizeof(double) * 4 * T))) return rc;
+        NEGF_HIP_TRY(hipMemcpy(d_K.p, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_ptrU.p, hU.data(), sizeof(double*) * T, hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_ptrUc.p, hUc.data(), sizeof(double*) * T, hipMemcpyHostToDevice));
+        if ((rc = fill_device(d_Uext.as<double>(), (long long)N * T, 0.0, st))) return rc;
+        if ((rc = fill_device(d_state.as<double>(), (long long)(state_stride * T), 0.0, st))) return rc;
+        for (int t = 0; t < T; ++t)
+            if ((rc = fill_device(d_state.as<double>() + t * state_stride, N, p.NS_initial_deposit_value, st)))
+                return rc;
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        return NEGF_OK;
+    }
+    BroydenDev base() const
+    {
+        BroydenDev b;
+        double* s0 = d_state.as<double>();
+        b.n_in = s0;
+        b.n_prev = s0 + N;
+        b.F = s0 + 2 * (size_t)N;
+        b.dF = s0 + 3 * (size_t)N;
+        b.Norm = s0 + 4 * (size_t)N;
+        b.sum_vec = s0 + 5 * (size_t)N;
+        double* h = d_hist.as<double>();
+        b.V = h;
+        b.W = h + (size_t)(M + 1) * N;
+        b.intermed = h + 2 * (size_t)(M + 1) * N;
+        b.out = d_out.as<double>();
+        return b;
+    }
+    int enqueue_iteration()
+    {
+        dim3 g((N + 31) / 32, (T + 31) / 32);
+        hipLaunchKernelGGL(dgemm_kx_kernel, g, dim3(256), 0, st, N, T, d_K.as<double>(), base().n_in, state_stride,
+                           d_KN.as<double>());
+        const long long NT = (long long)N * T;
+        hipLaunchKernelGGL(batch_potential_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
+                           has_ext ? d_Uext.as<double>() : nullptr, d_V.as<double>(), d_KN.as<double>(),
+                           d_ptrU.as<double*>(), d_ptrUc.as<double*>(), d_U.as<double>());
+        NEGF_HIP_TRY(hipGetLastError());
+        int rc = batch_enqueue(batch, d_Q.as<double>(), st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(broyden_batch_kernel, dim3(T), dim3(kBroydenThreads), 0, st, N, M, p.Broyden_fraction,
+                           p.Broyden_norm_type, d_Q.as<double>(), base(), state_stride, hist_stride,
+                           d_steps.as<int>(), d_active.as<int>());
+        NEGF_HIP_TRY(hipGetLastError());
+        NEGF_HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, sizeof(double) * 4 * T, hipMemcpyDeviceToHost, st));
+        return NEGF_OK;
+    }
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+    }
+    // Run every tube to Broyden_Norm <= Broyden_max_norm (or its step
+    // limit); per-tube iteration counts and final norms.
+    int run(const double* V, double* U_out, double* Q_out, int* iters, double* norms, int* total_iters)
+    {
+        int rc;
+        NEGF_HIP_TRY(hipMemcpyAsync(d_V.p, V, sizeof(double) * T, hipMemcpyHostToDevice, st));
+        std::vector<int> steps(T, 1), active(T, 1), keep(T, 0), done_it(T, -1);
+        std::vector<double> fin(T, 0.0);
+        NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        int it = 0, remaining = T;
+        bool overflow = false;
+        while (remaining > 0) {
+            if (it > M + 1) {
+                overflow = true;
+                break;
+            }
+            bool rebuilt = false;
+            if ((rc = batch_prepare(batch, it, &rebuilt))) return rc;
+            if (use_graph) {
+                if (!gexec || rebuilt) {
+                    drop_graph();
+                    NEGF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    int erc = enqueue_iteration();
+                    hipGraph_t g = nullptr;
+                    hipError_t e = hipStreamEndCapture(st, &g);
+                    if (erc) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return erc;
+                    }
+                    if (e != hipSuccess) {
+                        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+                        return NEGF_EDEVICE;
+                    }
+                    e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (e != hipSuccess) {
+                        gexec = nullptr;
+                        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+                        return NEGF_EDEVICE;
+                    }
+                }
+                NEGF_HIP_TRY(hipGraphLaunch(gexec, st));
+            } else {
+                if ((rc = enqueue_iteration())) return rc;
+            }
+            NEGF_HIP_TRY(hipStreamSynchronize(st));
+            const double* o = h_out.as<double>();
+            bool any = false;
+            for (int t = 0; t < T; ++t) {
+                keep[t] = 0;
+                if (!active[t]) continue;
+                if (o[4 * t + 3] != 0.0) {
+                    overflow = true;
+                    active[t] = 0;
+                    --remaining;
+                    continue;
+                }
+                if (o[4 * t] <= p.Broyden_max_norm) {
+                    keep[t] = 1;
+                    active[t] = 0;
+                    done_it[t] = it + 1;
+                    fin[t] = o[4 * t];
+                    --remaining;
+                    any = true;
+                }
+            }
+            if (any) {
+                NEGF_HIP_TRY(hipMemcpyAsync(d_keep.p, keep.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+                const long long NT = (long long)N * T;
+                hipLaunchKernelGGL(batch_keep_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
+                                   d_keep.as<int>(), d_U.as<double>(), d_Q.as<double>(), d_Uo.as<double>(),
+                                   d_Qo.as<double>());
+                NEGF_HIP_TRY(hipGetLastError());
+                NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+            }
+            ++it;
+        }
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        if (U_out) NEGF_HIP_TRY(hipMemcpy(U_out, d_Uo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        if (Q_out) NEGF_HIP_TRY(hipMemcpy(Q_out, d_Qo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        if (iters) std::copy(done_it.begin(), done_it.end(), iters);
+        if (norms) std::copy(fin.begin(), fin.end(), norms);
+        if (total_iters) *total_iters = it;
+        if (overflow) {
+            set_error("Broyden_Step has exceeded the Broyden_Threshold_MaxStep for some tube");
+            return NEGF_ENONCONV;
+        }
+        return NEGF_OK;
+    }
+
+    negf_batch* batch = nullptr;
+    int T = 0, N = 0, M = 0;
+    negf_scf_params p{};
+    std::vector<double> K;
+    std::vector<double*> hU, hUc;
+    size_t state_stride = 0, hist_stride = 0;
+    DevBuf d_K, d_Uext, d_V, d_KN, d_U, d_Q, d_Uo, d_Qo, d_ptrU, d_ptrUc, d_state, d_hist, d_out, d_steps,
+        d_active, d_keep;
+    HostBuf h_out;
+    bool has_ext = false, use_graph = true;
+    hipStream_t st = nullptr;
+    hipGraphExec_t gexec = nullptr;
+};
 
 }  // namespace negf
 
@@ -516,6 +835,106 @@ int negf_scf_bias_point(negf_scf* h, double V_gate, double* U_out, double* Q_out
     if ((rc = s.br.reset(s.p.reset_with_previous_charge_distribution, s.p.NS_initial_deposit_value, s.st))) return rc;
     NEGF_HIP_TRY(hipStreamSynchronize(s.st));
     return NEGF_OK;
+}
+
+struct negf_batch_scf {
+    BatchScf s;
+};
+
+int negf_batch_scf_create(negf_batch* b, negf_tube* const* tubes, int ntubes, const negf_scf_params* p,
+                          negf_batch_scf** out)
+{
+    clear_error();
+    if (!b || !tubes || ntubes <= 0 || !p || !out || p->eps_r <= 0 || p->Broyden_max_norm <= 0 ||
+        p->Broyden_threshold_maxstep < 0) {
+        set_error("invalid batched SCF arguments");
+        return NEGF_EARG;
+    }
+    negf_batch_scf* h = new negf_batch_scf;
+    int rc = h->s.init(b, tubes, ntubes, *p);
+    if (rc) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return NEGF_OK;
+}
+
+int negf_batch_scf_destroy(negf_batch_scf* s)
+{
+    if (!s) return NEGF_EARG;
+    delete s;
+    return NEGF_OK;
+}
+
+int negf_batch_scf_set_external_potential(negf_batch_scf* s, const double* U_ext)
+{
+    clear_error();
+    if (!s) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    if (!U_ext) {
+        s->s.has_ext = false;
+        return NEGF_OK;
+    }
+    NEGF_HIP_TRY(hipMemcpy(s->s.d_Uext.p, U_ext, sizeof(double) * s->s.N * s->s.T, hipMemcpyHostToDevice));
+    s->s.has_ext = true;
+    return NEGF_OK;
+}
+
+int negf_batch_scf_set_input_charge(negf_batch_scf* s, const double* n_in)
+{
+    clear_error();
+    if (!s || !n_in) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    BatchScf& b = s->s;
+    for (int t = 0; t < b.T; ++t)
+        NEGF_HIP_TRY(hipMemcpy(b.d_state.as<double>() + t * b.state_stride, n_in + (size_t)t * b.N,
+                               sizeof(double) * b.N, hipMemcpyHostToDevice));
+    return NEGF_OK;
+}
+
+int negf_batch_scf_get_input_charge(const negf_batch_scf* s, double* n_in)
+{
+    clear_error();
+    if (!s || !n_in) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    const BatchScf& b = s->s;
+    NEGF_HIP_TRY(hipStreamSynchronize(b.st));
+    for (int t = 0; t < b.T; ++t)
+        NEGF_HIP_TRY(hipMemcpy(n_in + (size_t)t * b.N, b.d_state.as<double>() + t * b.state_stride,
+                               sizeof(double) * b.N, hipMemcpyDeviceToHost));
+    return NEGF_OK;
+}
+
+int negf_batch_scf_use_graph(negf_batch_scf* s, int on)
+{
+    if (!s) return NEGF_EARG;
+    s->s.use_graph = on != 0;
+    if (!on) s->s.drop_graph();
+    return NEGF_OK;
+}
+
+int negf_batch_scf_run(negf_batch_scf* s, const double* V_gate, double* U_out, double* Q_out, int* iters,
+                       double* final_norms, int* total_iterations)
+{
+    clear_error();
+    if (!s || !V_gate) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    // each run starts a fresh Broyden history (Reset_Broyden_Parallel), keeping n_in
+    BatchScf& b = s->s;
+    int rc = fill_device(b.d_hist.as<double>(), (long long)(b.hist_stride * b.T), 0.0, b.st);
+    if (rc) return rc;
+    for (int t = 0; t < b.T; ++t)
+        if ((rc = fill_device(b.d_state.as<double>() + t * b.state_stride + b.N, 5LL * b.N, 0.0, b.st))) return rc;
+    return b.run(V_gate, U_out, Q_out, iters, final_norms, total_iterations);
 }
 
 }  // extern "C"

@@ -215,6 +215,59 @@ class Batch:
         return rc
 
 
+class BatchScf:
+    """Batched SCF of equal tubes sharing one electrostatic kernel."""
+
+    def __init__(self, batch, params=None, **keys):
+        self.lib = _lib.load()
+        self.batch = batch
+        self.T = len(batch.tubes)
+        self.N = batch.tubes[0].N
+        self.params = params if params is not None else scf_params(**keys)
+        arr = (C.c_void_p * self.T)(*[t.h for t in batch.tubes])
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_batch_scf_create(batch.h, arr, self.T, C.byref(self.params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.negf_batch_scf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_external_potential(self, U_ext):
+        self._U_ext = np.ascontiguousarray(U_ext, np.float64).reshape(self.T * self.N)
+        _lib.check(self.lib.negf_batch_scf_set_external_potential(self.h, _dp(self._U_ext)))
+
+    def set_input_charge(self, n_in):
+        n = np.ascontiguousarray(n_in, np.float64).reshape(self.T * self.N)
+        _lib.check(self.lib.negf_batch_scf_set_input_charge(self.h, _dp(n)))
+
+    def get_input_charge(self):
+        n = np.zeros(self.T * self.N)
+        _lib.check(self.lib.negf_batch_scf_get_input_charge(self.h, _dp(n)))
+        return n.reshape(self.T, self.N)
+
+    def use_graph(self, on=True):
+        _lib.check(self.lib.negf_batch_scf_use_graph(self.h, int(on)))
+
+    def run(self, V_gate):
+        V = np.ascontiguousarray(V_gate, np.float64)
+        U = np.zeros(self.T * self.N)
+        Q = np.zeros(self.T * self.N)
+        it = np.zeros(self.T, np.int32)
+        nrm = np.zeros(self.T)
+        tot = C.c_int()
+        _lib.check(self.lib.negf_batch_scf_run(self.h, _dp(V), _dp(U), _dp(Q), _ip(it), _dp(nrm),
+                                               C.byref(tot)))
+        return U.reshape(self.T, self.N), Q.reshape(self.T, self.N), it, nrm, tot.value
+
+
 class Broyden:
     """Broyden modified-second mixer on the GPU (Broyden_Second_Parallel.cpp)."""
 

@@ -294,6 +294,30 @@ int negf_scf_set_external_potential(negf_scf *s, const double *U_ext);
 /* Dense kernel K (N x N, row-major, V per e) used by the stand-in. */
 int negf_scf_kernel(const negf_scf *s, double *K);
 
+/* Batched SCF of equal tubes (no reference counterpart: one SCF loop per
+ * tube there).  The tubes of batch b (same order) share one dense
+ * electrostatic kernel K (N x N, as negf_scf); each tube t has its own
+ * external potential U_ext[t], metal voltage V_gate[t] and Broyden history.
+ * An iteration is one FP64 GEMM for all tubes, one batched NEGF solve and one
+ * Broyden workgroup per tube, replayed from a captured hipGraph; a tube stops
+ * iterating once its Broyden_Norm <= Broyden_max_norm.  Arrays are host,
+ * tube-major ([T][N]). */
+typedef struct negf_batch_scf negf_batch_scf;
+int negf_batch_scf_create(negf_batch *b, negf_tube *const *tubes, int ntubes,
+                          const negf_scf_params *p, negf_batch_scf **out);
+int negf_batch_scf_destroy(negf_batch_scf *s);
+int negf_batch_scf_set_external_potential(negf_batch_scf *s, const double *U_ext);
+/* Broyden input charge (initial guess) of every tube, and read-back. */
+int negf_batch_scf_set_input_charge(negf_batch_scf *s, const double *n_in);
+int negf_batch_scf_get_input_charge(const negf_batch_scf *s, double *n_in);
+int negf_batch_scf_use_graph(negf_batch_scf *s, int on);
+/* Run all tubes to convergence from the present input charge (fresh Broyden
+ * history); U_out, Q_out: [T][N] at each tube's converged iteration; iters,
+ * final_norms: [T]; total_iterations: batched iterations run. */
+int negf_batch_scf_run(negf_batch_scf *s, const double *V_gate, double *U_out,
+                       double *Q_out, int *iters, double *final_norms,
+                       int *total_iterations);
+
 /* ------------------------------------------------------------------------
  * Output writers (NEGF_Common.cpp:1343-1418, 3044-3112, 4223-4254):
  * <prefix>_Qout.dat, <prefix>_U.dat, <prefix>_norm.dat in the reference's

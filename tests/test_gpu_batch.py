@@ -114,3 +114,39 @@ def test_batch_long_uneven_tubes(dev):
         qj, _ = tubes[j].solve(1)
         assert rel(Q[batch.offsets[j]:batch.offsets[j + 1]], qj) < 1e-12
     batch.close()
+
+
+def test_batched_scf_matches_oracle_per_tube(dev):
+    T, cells = 4, 40
+    tubes = [negf.Tube(num_unitcells=cells, num_modes=1, contact_mu=(-1.0, -1.0),
+                       eq_integration_pts=(30, 30, 64), noneq_integration_pts=(200,),
+                       E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0) for _ in range(T)]
+    N = tubes[0].N
+    batch = negf.Batch(tubes)
+    sp = dict(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+    scf = negf.BatchScf(batch, Broyden_fraction=0.1, Broyden_max_norm=1e-8,
+                        Broyden_threshold_maxstep=200, **sp)
+    U_ext = np.stack([gaussian_bumps(N, seed=20 + t, amp=0.3) for t in range(T)])
+    V = np.array([0.0, 0.25, 0.5, 0.8])
+    scf.set_external_potential(U_ext)
+    U, Q, it, nrm, tot = scf.run(V)
+    assert np.all(nrm <= 1e-8) and tot == it.max()
+    _, betas, degs = tubes[0].modes()
+    K = O.coulomb_kernel(tubes[0].positions(), **sp)
+    for t in range(T):
+        ot = O.make_tube(N, list(betas), list(degs), mu=(-1.0, -1.0), eq_pts=(30, 30, 64),
+                         noneq_pts=(200,), eta=1e-5, E_valence_min=-10.0, E_pole_max=3.0)
+        r0, _ = O.rho0(ot)
+        ob = O.Broyden(N, 0.1, 200, 0)
+        Uo, Qo, ito, nrmo, st = O.scf_bias_point(ot, r0, K, V[t], ob, max_norm=1e-8,
+                                                 U_ext=U_ext[t])
+        assert st == 0
+        assert np.abs(U[t] - Uo).max() < 1e-6, t
+        assert abs(int(it[t]) - ito) <= 1, (t, it[t], ito)
+    # the graph path and the eager path agree
+    scf.use_graph(False)
+    scf.set_input_charge(np.zeros((T, N)))
+    U2, _, it2, _, _ = scf.run(V)
+    assert np.abs(U2 - U).max() < 1e-10
+    scf.close()
+    batch.close()
