@@ -129,6 +129,18 @@ SIGNATURES = {
     "negf_batch_scf_get_input_charge": (C.c_int, [vp, dp]),
     "negf_batch_scf_use_graph": (C.c_int, [vp, C.c_int]),
     "negf_batch_scf_run": (C.c_int, [vp, dp, dp, dp, ip, dp, ip]),
+    "negf_surrogate_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(vp)]),
+    "negf_surrogate_destroy": (C.c_int, [vp]),
+    "negf_surrogate_num_params": (C.c_int, [vp]),
+    "negf_surrogate_num_sites": (C.c_int, [vp]),
+    "negf_surrogate_get_params": (C.c_int, [vp, C.POINTER(C.c_float), dp]),
+    "negf_surrogate_set_params": (C.c_int, [vp, C.POINTER(C.c_float), dp]),
+    "negf_surrogate_forward": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
+    "negf_surrogate_jvp": (C.c_int, [vp, C.c_int, vp, vp, vp, vp, vp]),
+    "negf_surrogate_gradient": (C.c_int, [vp, C.c_int, vp, vp, vp, C.POINTER(C.c_float), dp]),
+    "negf_surrogate_fit_output_scale": (C.c_int, [vp, C.c_int, vp]),
+    "negf_surrogate_train": (C.c_int, [vp, C.c_int, vp, vp, vp, C.c_int, C.c_int, C.c_double,
+                                       C.c_double, C.c_ulonglong, dp]),
     "negf_broyden_create": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(vp)]),
     "negf_broyden_destroy": (C.c_int, [vp]),
     "negf_broyden_set_nin": (C.c_int, [vp, dp]),

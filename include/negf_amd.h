@@ -319,6 +319,49 @@ int negf_batch_scf_run(negf_batch_scf *s, const double *V_gate, double *U_out,
                        int *total_iterations);
 
 /* ------------------------------------------------------------------------
+ * Charge surrogate (no reference counterpart; a new component next to the
+ * NEGF charge path).  A 1D-convolutional network maps a tube's potential
+ * U (eV, N sites) and one per-sample feature f (e.g. the drain bias) to an
+ * estimate of the NEGF induced charge Q: 6 layers (3 -> 64, 4 x 64 -> 64,
+ * 64 -> 1), kernel width 9, tanh, BF16 weights and activations with FP32
+ * accumulation on MFMA.  It is trained on the device from (U, f, Q) pairs
+ * that the NEGF solver produced, and used only to seed and precondition
+ * the batched SCF.  Array arguments named d_* are device pointers,
+ * sample-major ([B][N] doubles, f: [B] doubles or NULL); stream NULL means
+ * the surrogate's own stream.
+ * ------------------------------------------------------------------------ */
+typedef struct negf_surrogate negf_surrogate;
+int negf_surrogate_create(int N, unsigned long long seed, negf_surrogate **out);
+int negf_surrogate_destroy(negf_surrogate *s);
+int negf_surrogate_num_params(const negf_surrogate *s);
+int negf_surrogate_num_sites(const negf_surrogate *s);
+/* Flat FP32 parameters (per layer W[cout][9][cin] then b[cout]; cin 32 for
+ * the first layer, cout 32 for the last, padding ignored) and the output
+ * normalisation norm = {mean_Q, scale_Q}. */
+int negf_surrogate_get_params(negf_surrogate *s, float *w, double *norm);
+int negf_surrogate_set_params(negf_surrogate *s, const float *w, const double *norm);
+/* Q estimate of B samples. */
+int negf_surrogate_forward(negf_surrogate *s, int B, const double *d_U,
+                           const double *d_feat, double *d_Q, void *stream);
+/* Tangent dQ/dU . V at U (B samples). */
+int negf_surrogate_jvp(negf_surrogate *s, int B, const double *d_U,
+                       const double *d_feat, const double *d_V,
+                       double *d_out, void *stream);
+/* Normalised mean-squared loss against d_Q and its parameter gradient
+ * (host float[num_params]). */
+int negf_surrogate_gradient(negf_surrogate *s, int B, const double *d_U,
+                            const double *d_feat, const double *d_Q,
+                            float *grad, double *loss);
+/* mean_Q and scale_Q from the charges of nsamples samples. */
+int negf_surrogate_fit_output_scale(negf_surrogate *s, int nsamples, const double *d_Q);
+/* Adam over shuffled minibatches; the learning rate goes linearly from lr
+ * to lr_end; last_loss: the loss of the last minibatch. */
+int negf_surrogate_train(negf_surrogate *s, int nsamples, const double *d_U,
+                         const double *d_feat, const double *d_Q, int epochs,
+                         int batch, double lr, double lr_end,
+                         unsigned long long seed, double *last_loss);
+
+/* ------------------------------------------------------------------------
  * Output writers (NEGF_Common.cpp:1343-1418, 3044-3112, 4223-4254):
  * <prefix>_Qout.dat, <prefix>_U.dat, <prefix>_norm.dat in the reference's
  * column formats.

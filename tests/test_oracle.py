@@ -392,3 +392,69 @@ def test_current_is_zero_in_equilibrium():
     t = O.make_tube(12, [BETA6], [2], mu=(-1.0, -1.0))
     I, _ = O.current(t, np.zeros(12), (0.0, 0.0))
     assert I[0] == 0.0 and I[1] == 0.0
+
+
+# ---------------------------------------------------------------- surrogate CPU model
+def test_surrogate_model_gradient_matches_central_differences():
+    from oracle import surrogate_oracle as S
+    p = S.init_params(1, np.float64)
+    rng = np.random.default_rng(0)
+    B, N = 2, 40
+    U = rng.uniform(-0.5, 0.5, (B, N))
+    Q = rng.standard_normal((B, N))
+    f = rng.uniform(0, 0.5, B)
+    _, g = S.loss_and_grad(p, U, Q, 0.1, 2.0, f, bf=False)
+    lay, n = S.layout()
+    h = 1e-6
+    for l, (wo, bo) in enumerate(lay):
+        W = np.zeros((S.cout(l), S.KW, S.cin(l)), bool)
+        W[:S.real_cout(l), :, :S.real_cin(l)] = True
+        idx = list(wo + rng.choice(np.flatnonzero(W.ravel()), 3, replace=False)) + [bo]
+        for j in idx:
+            e = np.zeros(n)
+            e[j] = h
+            fd = (S.loss_and_grad(p + e, U, Q, 0.1, 2.0, f, bf=False)[0]
+                  - S.loss_and_grad(p - e, U, Q, 0.1, 2.0, f, bf=False)[0]) / (2 * h)
+            assert abs(g[j] - fd) <= 1e-6 * max(abs(fd), 1e-3), (l, j, g[j], fd)
+
+
+def test_surrogate_model_tangent_matches_central_differences():
+    from oracle import surrogate_oracle as S
+    p = S.init_params(2, np.float64)
+    rng = np.random.default_rng(1)
+    U = rng.uniform(-0.5, 0.5, (2, 30))
+    V = rng.standard_normal((2, 30))
+    jv = S.jvp(p, U, V, 0.7, None, bf=False)
+    h = 1e-6
+    fd = 0.7 * (S.forward(p, U + h * V, None, bf=False)[0] - S.forward(p, U - h * V, None, bf=False)[0]) / (2 * h)
+    assert np.abs(jv - fd).max() < 1e-7 * np.abs(fd).max()
+
+
+def test_surrogate_model_conv_is_a_banded_matrix_product():
+    # conv9 against an explicit (N*ci) x (N*co) band matrix: zero padding,
+    # tap order n + tap - 4, and the adjoint as its transpose
+    from oracle import surrogate_oracle as S
+    rng = np.random.default_rng(2)
+    N, ci, co = 12, 3, 2
+    W = rng.standard_normal((co, S.KW, ci))
+    x = rng.standard_normal((1, N, ci))
+    M = np.zeros((N * co, N * ci))
+    for n in range(N):
+        for tap in range(S.KW):
+            m = n + tap - S.HALO
+            if 0 <= m < N:
+                M[n * co:(n + 1) * co, m * ci:(m + 1) * ci] += W[:, tap, :]
+    z = S.conv(x, W)
+    assert np.allclose(z.ravel(), M @ x.ravel())
+    dz = rng.standard_normal((1, N, co))
+    assert np.allclose(S.conv_T(dz, W).ravel(), M.T @ dz.ravel())
+
+
+def test_bf16_rounding_is_nearest_even():
+    from oracle import surrogate_oracle as S
+    one = 1.0
+    ulp = 2.0 ** -7          # bf16 spacing at 1
+    assert S.bf16(one + ulp / 2) == one            # tie -> even (1.0)
+    assert S.bf16(one + 1.5 * ulp) == one + 2 * ulp  # tie -> even
+    assert S.bf16(one + 0.51 * ulp) == one + ulp
+    assert S.bf16(-3.0) == -3.0
