@@ -157,6 +157,56 @@ def batch_leg(dev, steps=10):
     return out
 
 
+def block_leg(steps=3, with_cpu=True):
+    """Dense-block chain: atomistic (16,0) tube in blocks of two rings (32
+    orbitals), 2000 blocks, 1024 energies (512 on a contour arc, 512 on the
+    real axis [-3, 3] eV with eta = 1e-5 eV), on-site disorder N(0, 0.05 eV)
+    per orbital (seed 4) plus a smooth potential; Sancho-Rubio leads to
+    1e-12.  Algorithmic flops per energy-block: 36 m^3 (forward: g T, T^T (gT),
+    inverse; backward: g T, X G, Y X^T; real H0 and T)."""
+    from electrostatic_amd import negf
+    from oracle import block_oracle as B
+    nb, nE, m = 2000, 1024, 32
+    H0, T = B.zigzag_blocks(16, 2.5)
+    rng = np.random.default_rng(4)
+    v = rng.normal(0.0, 0.05, (nb, m)) + 0.2 * np.sin(np.linspace(0, np.pi, nb))[:, None]
+    E = np.concatenate([np.linspace(-3, 3, nE // 2) + 1e-5j,
+                        -1.0 + 1.5 * np.exp(1j * np.linspace(0.01, np.pi - 0.01, nE - nE // 2))])
+    w = np.full(nE, 1.0 / nE, np.complex128)
+    c = negf.BlockChain(nb, H0, T)
+    c.set_potential(v, (0.0, 0.0))
+    acc, _, it = c.rgf(E, w)
+    sr_ms, rgf_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        acc, _, it = c.rgf(E, w)
+        a, b = c.last_time()
+        sr_ms.append(a)
+        rgf_ms.append(b)
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    c.close()
+    if not np.all(np.isfinite(acc)):
+        raise RuntimeError("non-finite block charge")
+    flops = 36 * m ** 3 * nb * nE
+    rg = float(np.median(rgf_ms))
+    out = {"workload": "(16,0) tube, 2000 blocks of 32 orbitals, 1024 energies, disorder N(0, 0.05 eV) "
+                       "+ smooth U, Sancho-Rubio to 1e-12",
+           "ms_per_evaluation": wall, "ms_sancho_rubio": float(np.median(sr_ms)), "ms_block_rgf": rg,
+           "sancho_rubio_iterations_max": int(it.max()),
+           "energy_blocks_per_s": nb * nE / (wall * 1e-3),
+           "rgf_tflops": flops / (rg * 1e-3) / 1e12,
+           "rgf_frac_fp64_peak": flops / (rg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+    if with_cpu:
+        # numpy oracle on 4 energies of the same chain
+        sample = E[::nE // 4][:4]
+        t0 = time.perf_counter()
+        B.accumulate(sample, np.ones(4), H0, T, v, (0.0, 0.0))
+        dt = time.perf_counter() - t0
+        out["cpu_energy_blocks_per_s"] = 4 * nb / dt
+        out["cpu_kind"] = "port (numpy block oracle, 4 energies x 2000 blocks)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +217,8 @@ def main():
                     help="RGF segment length (0: library default)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the secondary 64-tube batched-solve measurement")
+    ap.add_argument("--no-block", action="store_true",
+                    help="skip the dense-block (32-orbital) chain leg")
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the secondary SCF gate-sweep measurement")
     ap.add_argument("--write-diag-g", action="store_true",
@@ -291,6 +343,8 @@ def main():
         out["scf"] = scf_sweep(with_cpu=not args.no_cpu_baseline)
     if not args.no_batch:
         out["batch"] = batch_leg(dev)
+    if not args.no_block:
+        out["block"] = block_leg(with_cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

@@ -129,6 +129,13 @@ SIGNATURES = {
     "negf_batch_scf_get_input_charge": (C.c_int, [vp, dp]),
     "negf_batch_scf_use_graph": (C.c_int, [vp, C.c_int]),
     "negf_batch_scf_run": (C.c_int, [vp, dp, dp, dp, ip, dp, ip]),
+    "negf_block_chain_create": (C.c_int, [C.c_int, C.c_int, dp, dp, C.POINTER(vp)]),
+    "negf_block_chain_destroy": (C.c_int, [vp]),
+    "negf_block_chain_set_potential": (C.c_int, [vp, dp, dp]),
+    "negf_block_rgf": (C.c_int, [vp, C.c_int, vp, vp, C.c_double, C.c_int, dp, vp, ip]),
+    "negf_block_self_energies": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_int, dp, ip]),
+    "negf_block_probe": (C.c_int, [C.c_int, vp, vp, vp]),
+    "negf_block_last_time": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "negf_surrogate_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(vp)]),
     "negf_surrogate_destroy": (C.c_int, [vp]),
     "negf_surrogate_num_params": (C.c_int, [vp]),

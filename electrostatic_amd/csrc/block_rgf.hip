@@ -1,0 +1,735 @@
+// block_rgf.hip -- dense-block NEGF path on FP64 MFMA: Sancho-Rubio lead
+// surface Green's functions and the block recursive Green's function sweep
+// (diag G^R) for chains of 32 x 32 blocks.
+//
+// The reference declares this path but leaves it stubbed (2-D MatrixBlock
+// algebra in Matrix_Block_Util.H:123-135, 431-438, 684-692; decimation
+// DecimationTechnique NEGF_Common.cpp:4095-4153 only for 1-D blocks); the
+// mathematics follows the scalar sweep of Compute_RhoEq
+// (NEGF_Common.cpp:3667-3754) with blocks in place of scalars.
+//
+// Material: every block has the same real symmetric on-site matrix H0 plus a
+// per-orbital real potential v_b, and every block couples to the next by the
+// same real matrix T = H_{b,b+1}; the leads continue the chain with the
+// contact potential added to H0.  All left-connected and full block Green's
+// functions are then complex symmetric, which the backward step uses:
+//   G_bb = g_b + X G_{b+1,b+1} X^T,   X = g_b T.
+//
+// Layout: one workgroup (4 wavefronts) per energy; each wavefront owns one
+// 16 x 16 quadrant of every 32 x 32 product and runs it on
+// v_mfma_f64_16x16x4f64 (lane l: A[i][k] with i = l % 16, k = l / 16;
+// B[k][j] with j = l % 16; result register v holds D[l / 16 + 4 v][l % 16],
+// as a GPU probe against numpy established for this FP64 instruction).  Complex
+// matrices are two real planes in LDS (row stride 33 doubles).  Inverses are
+// in-place Gauss-Jordan with partial pivoting over the whole workgroup.  The
+// forward sweep stores each g_b (16 KiB) to a device scratch array that the
+// backward sweep reads back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "negf_internal.h"
+
+namespace negf {
+
+typedef __attribute__((ext_vector_type(4))) double d4;
+
+constexpr int BM = 32;               // orbitals per block
+constexpr int BLD = 33;              // LDS row stride (doubles)
+constexpr int BPL = BM * BLD;        // doubles per LDS plane
+constexpr int BTHREADS = 256;        // 4 wavefronts, one quadrant each
+constexpr int BGL = BM * BM;         // doubles per plane in global memory (row-major, unpadded)
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+struct Quad {
+    int lane, qi, qj;
+    __device__ Quad()
+    {
+        lane = threadIdx.x & 63;
+        const int w = threadIdx.x >> 6;
+        qi = w >> 1;
+        qj = w & 1;
+    }
+    __device__ int row(int v) const { return qi * 16 + (lane >> 4) + 4 * v; }
+    __device__ int col() const { return qj * 16 + (lane & 15); }
+};
+
+// (cre, cim) += A B over this wavefront's quadrant; AC/BC: complex operand;
+// TA: A(i,k) is read at A[k][i] (A stored transposed); TB: B(k,j) at B[j][k].
+template <bool AC, bool BC, bool TA, bool TB>
+__device__ __forceinline__ void mm(d4& cre, d4& cim, const double* Are, const double* Aim, const double* Bre,
+                                   const double* Bim, const Quad& q)
+{
+    const int i = q.qi * 16 + (q.lane & 15), j = q.qj * 16 + (q.lane & 15), kk = q.lane >> 4;
+#pragma unroll
+    for (int k0 = 0; k0 < BM; k0 += 4) {
+        const int k = k0 + kk;
+        const int ia = TA ? k * BLD + i : i * BLD + k;
+        const int ib = TB ? j * BLD + k : k * BLD + j;
+        const double ar = Are[ia], br = Bre[ib];
+        if (AC && BC) {
+            const double ai = Aim[ia], bi = Bim[ib];
+            cre = mfma64(ar, br, cre);
+            cre = mfma64(-ai, bi, cre);
+            cim = mfma64(ar, bi, cim);
+            cim = mfma64(ai, br, cim);
+        } else if (AC) {
+            const double ai = Aim[ia];
+            cre = mfma64(ar, br, cre);
+            cim = mfma64(ai, br, cim);
+        } else {
+            const double bi = Bim[ib];
+            cre = mfma64(ar, br, cre);
+            cim = mfma64(ar, bi, cim);
+        }
+    }
+}
+
+__device__ __forceinline__ void store_quad(double* Mre, double* Mim, const d4& cre, const d4& cim, const Quad& q)
+{
+    const int j = q.col();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        Mre[q.row(v) * BLD + j] = cre[v];
+        Mim[q.row(v) * BLD + j] = cim[v];
+    }
+}
+
+__device__ __forceinline__ double mag2(double re, double im) { return re * re + im * im; }
+
+// max that keeps a NaN (so that a NaN never reads as converged)
+__device__ __forceinline__ double nan_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// Pivot candidate key: |a|^2 (non-negative, so its bits order as an
+// unsigned integer) with the low 5 mantissa bits replaced by 31 - row, so
+// that the largest key is the largest magnitude and, among equal ones, the
+// smallest row.  0 marks "no candidate".
+__device__ __forceinline__ unsigned long long pivot_key(double m, int i)
+{
+    return ((unsigned long long)__double_as_longlong(m) & ~31ull) | (unsigned long long)(31 - i);
+}
+
+// Inverse of the complex matrix (Mre, Mim), in place, by Gauss-Jordan with
+// partial pivoting (row swaps, undone as column swaps at the end); (Wre, Wim)
+// is a second 32 x 32 buffer that the steps alternate with, so that one
+// barrier per step suffices.  Thread t owns column j = t % 32 of rows
+// t / 32 + 8 r (r = 0..3): each step reads the old matrix (the pivot row swap
+// folded into the reads) and writes its four new entries to the other
+// buffer.  The threads of the next pivot column leave one candidate key each
+// in keys[2][8], so the pivot search is a max over 8 integers.
+__device__ void gj_inverse(double* Mre, double* Mim, double* Wre, double* Wim, unsigned long long* keys, int* perm)
+{
+    const int tid = threadIdx.x, j = tid & 31, g = tid >> 5;
+    int roff[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) roff[r] = (g + 8 * r) * BLD;
+    if (j == 0) {
+        unsigned long long kb = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const unsigned long long kk = pivot_key(mag2(Mre[roff[r]], Mim[roff[r]]), g + 8 * r);
+            kb = kk > kb ? kk : kb;
+        }
+        keys[g] = kb;
+    }
+    __syncthreads();
+    for (int k = 0; k < BM; ++k) {
+        const double* Sre = (k & 1) ? Wre : Mre;
+        const double* Sim = (k & 1) ? Wim : Mim;
+        double* Dre = (k & 1) ? Mre : Wre;
+        double* Dim = (k & 1) ? Mim : Wim;
+        const unsigned long long* kin = keys + 8 * (k & 1);
+        unsigned long long kb = kin[0];
+#pragma unroll
+        for (int sl = 1; sl < 8; ++sl) {
+            const unsigned long long kk = kin[sl];
+            kb = kk > kb ? kk : kb;
+        }
+        const int p = (kb == 0) ? k : 31 - (int)(kb & 31ull);  // no candidate only for a NaN column
+        const int poff = p * BLD, koff = k * BLD;
+        const double ar = Sre[poff + k], ai = Sim[poff + k];
+        const double rr = Sre[poff + j], ri = Sim[poff + j];
+        double fr[4], fi[4], oR[4], oI[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int so = (g + 8 * r == p) ? koff : roff[r];  // rows k and p trade places
+            fr[r] = Sre[so + k];
+            fi[r] = Sim[so + k];
+            oR[r] = Sre[so + j];
+            oI[r] = Sim[so + j];
+        }
+        const double inv = 1.0 / mag2(ar, ai);
+        const double pr = ar * inv, pi = -ai * inv;  // 1 / pivot
+        const bool jk = (j == k);
+        const double nr = jk ? pr : rr * pr - ri * pi;  // new pivot row, column j
+        const double ni = jk ? pi : rr * pi + ri * pr;
+        unsigned long long kc = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = g + 8 * r;
+            const double a0 = jk ? 0.0 : oR[r], b0 = jk ? 0.0 : oI[r];
+            const double ur = a0 - (fr[r] * nr - fi[r] * ni);
+            const double ui = b0 - (fr[r] * ni + fi[r] * nr);
+            const double vr = (i == k) ? nr : ur;
+            const double vi = (i == k) ? ni : ui;
+            Dre[roff[r] + j] = vr;
+            Dim[roff[r] + j] = vi;
+            // candidates of the next pivot column (threads of column k + 1)
+            const unsigned long long kk = (i > k) ? pivot_key(mag2(vr, vi), i) : 0ull;
+            kc = kk > kc ? kk : kc;
+        }
+        if (j == k + 1) keys[8 * ((k + 1) & 1) + g] = kc;
+        if (tid == 0) perm[k] = p;
+        __syncthreads();
+    }
+    // undo the row swaps as column swaps (perm[31] first .. perm[0] last):
+    // column j of the result is column c of the present matrix
+    int c = j;
+    for (int k = 0; k < BM; ++k) {
+        const int pk = perm[k];
+        c = (c == k) ? pk : (c == pk ? k : c);
+    }
+    double vr[4], vi[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        vr[r] = Mre[roff[r] + c];
+        vi[r] = Mim[roff[r] + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        Mre[roff[r] + j] = vr[r];
+        Mim[roff[r] + j] = vi[r];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void load_real(double* dst, const double* __restrict__ src)
+{
+    for (int e = threadIdx.x; e < BM * BM; e += BTHREADS) dst[(e >> 5) * BLD + (e & 31)] = src[e];
+}
+
+__device__ __forceinline__ void load_cplx(double* dre, double* dim, const double* __restrict__ src)
+{
+    for (int e = threadIdx.x; e < BM * BM; e += BTHREADS) {
+        dre[(e >> 5) * BLD + (e & 31)] = src[e];
+        dim[(e >> 5) * BLD + (e & 31)] = src[BGL + e];
+    }
+}
+
+__device__ __forceinline__ void store_cplx(double* __restrict__ dst, const double* sre, const double* sim)
+{
+    for (int e = threadIdx.x; e < BM * BM; e += BTHREADS) {
+        dst[e] = sre[(e >> 5) * BLD + (e & 31)];
+        dst[BGL + e] = sim[(e >> 5) * BLD + (e & 31)];
+    }
+}
+
+// Sancho-Rubio decimation for one energy (blockIdx.x) and one lead
+// (blockIdx.y: 0 = left lead, surface = its last block, alpha = T^T;
+// 1 = right lead, surface = its first block, alpha = T), on-site H0 + Uc.
+// Iterates until max |alpha|, |beta| < tol (at most max_iter times); writes
+// the self-energy Sigma_L = T^T g T or Sigma_R = T g T^T (global complex
+// planes) and the iteration count (negative if not converged).
+__global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const double2* __restrict__ E,
+                                                                const double* __restrict__ H0g,
+                                                                const double* __restrict__ Tg, double Uc0, double Uc1,
+                                                                double tol, int max_iter, double* __restrict__ Sigma,
+                                                                int* __restrict__ iters)
+{
+    __shared__ double Tl[BPL], eps_s[2][BPL], eps[2][BPL], a[2][BPL], b[2][BPL], G[2][BPL], t1[2][BPL];
+    __shared__ unsigned long long keys[16];
+    __shared__ int perm[BM];
+    __shared__ double wmax[4];
+    const int e = blockIdx.x, side = blockIdx.y;
+    if (e >= nE) return;
+    const double2 z = E[e];
+    const double Er = z.x - (side == 0 ? Uc0 : Uc1), Ei = z.y;
+    const int tid = threadIdx.x;
+    const Quad q;
+    load_real(Tl, Tg);
+    for (int e2 = tid; e2 < BM * BM; e2 += BTHREADS) {
+        const int i = e2 >> 5, j = e2 & 31;
+        const double h = H0g[e2];
+        eps_s[0][i * BLD + j] = h;
+        eps_s[1][i * BLD + j] = 0.0;
+        eps[0][i * BLD + j] = h;
+        eps[1][i * BLD + j] = 0.0;
+        // alpha = H_{0,1} of the semi-infinite direction: T^T (left lead), T (right lead)
+        const double t = Tg[e2], tt = Tg[j * BM + i];
+        a[0][i * BLD + j] = side == 0 ? tt : t;
+        a[1][i * BLD + j] = 0.0;
+        b[0][i * BLD + j] = side == 0 ? t : tt;
+        b[1][i * BLD + j] = 0.0;
+    }
+    __syncthreads();
+    int it = 0;
+    bool conv = false;
+    while (it < max_iter) {
+        ++it;
+        for (int e2 = tid; e2 < BM * BM; e2 += BTHREADS) {
+            const int i = e2 >> 5, j = e2 & 31;
+            G[0][i * BLD + j] = (i == j ? Er : 0.0) - eps[0][i * BLD + j];
+            G[1][i * BLD + j] = (i == j ? Ei : 0.0) - eps[1][i * BLD + j];
+        }
+        __syncthreads();
+        gj_inverse(G[0], G[1], t1[0], t1[1], keys, perm);
+        // ga = G a, gb = G b
+        d4 r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0}, r2 = {0, 0, 0, 0}, i2 = {0, 0, 0, 0};
+        mm<true, true, false, false>(r1, i1, G[0], G[1], a[0], a[1], q);
+        mm<true, true, false, false>(r2, i2, G[0], G[1], b[0], b[1], q);
+        __syncthreads();
+        store_quad(G[0], G[1], r1, i1, q);    // G <- ga
+        store_quad(t1[0], t1[1], r2, i2, q);  // t1 <- gb
+        __syncthreads();
+        d4 agb_r = {0, 0, 0, 0}, agb_i = {0, 0, 0, 0}, bga_r = {0, 0, 0, 0}, bga_i = {0, 0, 0, 0};
+        d4 an_r = {0, 0, 0, 0}, an_i = {0, 0, 0, 0}, bn_r = {0, 0, 0, 0}, bn_i = {0, 0, 0, 0};
+        mm<true, true, false, false>(agb_r, agb_i, a[0], a[1], t1[0], t1[1], q);
+        mm<true, true, false, false>(bga_r, bga_i, b[0], b[1], G[0], G[1], q);
+        mm<true, true, false, false>(an_r, an_i, a[0], a[1], G[0], G[1], q);
+        mm<true, true, false, false>(bn_r, bn_i, b[0], b[1], t1[0], t1[1], q);
+        __syncthreads();
+        double m = 0.0;
+        const int j = q.col();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = q.row(v) * BLD + j;
+            eps_s[0][o] += agb_r[v];
+            eps_s[1][o] += agb_i[v];
+            eps[0][o] += agb_r[v] + bga_r[v];
+            eps[1][o] += agb_i[v] + bga_i[v];
+            a[0][o] = an_r[v];
+            a[1][o] = an_i[v];
+            b[0][o] = bn_r[v];
+            b[1][o] = bn_i[v];
+            m = nan_max(m, nan_max(mag2(an_r[v], an_i[v]), mag2(bn_r[v], bn_i[v])));
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = nan_max(m, __shfl_xor(m, off));
+        if (q.lane == 0) wmax[tid >> 6] = m;
+        __syncthreads();
+        const double mm4 = nan_max(nan_max(wmax[0], wmax[1]), nan_max(wmax[2], wmax[3]));
+        __syncthreads();
+        if (mm4 < tol * tol) {
+            conv = true;
+            break;
+        }
+    }
+    // surface Green's function g = (E - eps_s)^-1 into G
+    for (int e2 = tid; e2 < BM * BM; e2 += BTHREADS) {
+        const int i = e2 >> 5, j = e2 & 31;
+        G[0][i * BLD + j] = (i == j ? Er : 0.0) - eps_s[0][i * BLD + j];
+        G[1][i * BLD + j] = (i == j ? Ei : 0.0) - eps_s[1][i * BLD + j];
+    }
+    __syncthreads();
+    gj_inverse(G[0], G[1], t1[0], t1[1], keys, perm);
+    d4 r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
+    if (side == 0) {
+        // Sigma_L = T^T (g T)
+        mm<true, false, false, false>(r1, i1, G[0], G[1], Tl, nullptr, q);
+        __syncthreads();
+        store_quad(t1[0], t1[1], r1, i1, q);
+        __syncthreads();
+        r1 = d4{0, 0, 0, 0};
+        i1 = d4{0, 0, 0, 0};
+        mm<false, true, true, false>(r1, i1, Tl, nullptr, t1[0], t1[1], q);
+    } else {
+        // Sigma_R = T (g T^T)
+        mm<true, false, false, true>(r1, i1, G[0], G[1], Tl, nullptr, q);
+        __syncthreads();
+        store_quad(t1[0], t1[1], r1, i1, q);
+        __syncthreads();
+        r1 = d4{0, 0, 0, 0};
+        i1 = d4{0, 0, 0, 0};
+        mm<false, true, false, false>(r1, i1, Tl, nullptr, t1[0], t1[1], q);
+    }
+    double* out = Sigma + ((size_t)e * 2 + side) * 2 * BGL;
+    const int j = q.col();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        out[q.row(v) * BM + j] = r1[v];
+        out[BGL + q.row(v) * BM + j] = i1[v];
+    }
+    if (tid == 0) iters[e * 2 + side] = conv ? it : -it;
+}
+
+// Block RGF for energy e0 + blockIdx.x: forward sweep of left-connected
+// g_b (stored to gstore[blockIdx.x][b]), backward sweep of G_bb; writes
+// part[e][b*32 + o] = Im(coef_e G_e,bb,oo) and, if diag, G_e,bb,oo.
+__global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int nb, const double2* __restrict__ E,
+                                                             const double2* __restrict__ coef,
+                                                             const double* __restrict__ H0g,
+                                                             const double* __restrict__ Tg,
+                                                             const double* __restrict__ v,
+                                                             const double* __restrict__ Sigma,
+                                                             double* __restrict__ gstore, double* __restrict__ part,
+                                                             double2* __restrict__ diag)
+{
+    __shared__ double Tl[BPL], Hl[BPL], C[3][2][BPL];
+    __shared__ unsigned long long keys[16];
+    __shared__ int perm[BM];
+    const int el = blockIdx.x;
+    const int e = e0 + el;
+    if (e >= nE) return;
+    const int tid = threadIdx.x;
+    const Quad q;
+    const double Er = E[e].x, Ei = E[e].y;
+    const double cr = coef[e].x, ci = coef[e].y;
+    const double* SL = Sigma + (size_t)e * 4 * BGL;
+    const double* SR = SL + 2 * BGL;
+    double* gs = gstore + (size_t)el * (nb > 1 ? nb - 1 : 1) * 2 * BGL;  // g_0 .. g_{nb-2}
+    load_real(Tl, Tg);
+    load_real(Hl, H0g);
+    __syncthreads();
+    double* Cur[2] = {C[0][0], C[0][1]};
+    double* P[2] = {C[1][0], C[1][1]};
+    // ---- forward sweep
+    for (int b = 0; b < nb; ++b) {
+        d4 sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+        if (b > 0) {
+            // P = g_{b-1} T, S = T^T P
+            d4 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
+            mm<true, false, false, false>(pr, pi, Cur[0], Cur[1], Tl, nullptr, q);
+            store_quad(P[0], P[1], pr, pi, q);
+            __syncthreads();
+            mm<false, true, true, false>(sr, si, Tl, nullptr, P[0], P[1], q);
+            __syncthreads();
+        }
+        // A = E - H0 - diag(v_b) - S (- Sigma_L on the first block, - Sigma_R on the last)
+        const double* vb = v + (size_t)b * BM;
+        const int j = q.col();
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+            const int i = q.row(vv);
+            double ar = -Hl[i * BLD + j] - sr[vv];
+            double ai = -si[vv];
+            if (i == j) {
+                ar += Er - vb[i];
+                ai += Ei;
+            }
+            if (b == 0) {
+                ar -= SL[i * BM + j];
+                ai -= SL[BGL + i * BM + j];
+            }
+            if (b == nb - 1) {
+                ar -= SR[i * BM + j];
+                ai -= SR[BGL + i * BM + j];
+            }
+            Cur[0][i * BLD + j] = ar;
+            Cur[1][i * BLD + j] = ai;
+        }
+        __syncthreads();
+        gj_inverse(Cur[0], Cur[1], P[0], P[1], keys, perm);
+        if (b < nb - 1) store_cplx(gs + (size_t)b * 2 * BGL, Cur[0], Cur[1]);
+    }
+    // ---- backward sweep: Cur holds G_{nb-1} = g_{nb-1}
+    auto emit = [&](int b, const double* Gre, const double* Gim) {
+        if (tid < BM) {
+            const double gr = Gre[tid * BLD + tid], gi = Gim[tid * BLD + tid];
+            part[(size_t)e * nb * BM + (size_t)b * BM + tid] = cr * gi + ci * gr;
+            if (diag) diag[(size_t)e * nb * BM + (size_t)b * BM + tid] = make_double2(gr, gi);
+        }
+    };
+    emit(nb - 1, Cur[0], Cur[1]);
+    double* Gn[2] = {Cur[0], Cur[1]};
+    double* X[2] = {C[1][0], C[1][1]};
+    double* W[2] = {C[2][0], C[2][1]};
+    for (int b = nb - 2; b >= 0; --b) {
+        __syncthreads();
+        load_cplx(W[0], W[1], gs + (size_t)b * 2 * BGL);
+        __syncthreads();
+        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
+        mm<true, false, false, false>(xr, xi, W[0], W[1], Tl, nullptr, q);  // X = g_b T
+        store_quad(X[0], X[1], xr, xi, q);
+        __syncthreads();
+        d4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+        mm<true, true, false, false>(yr, yi, X[0], X[1], Gn[0], Gn[1], q);  // Y = X G_{b+1}
+        __syncthreads();
+        store_quad(Gn[0], Gn[1], yr, yi, q);  // Gn <- Y
+        __syncthreads();
+        d4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
+        mm<true, true, false, true>(gr, gi, Gn[0], Gn[1], X[0], X[1], q);  // Y X^T
+        const int j = q.col();
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+            const int o = q.row(vv) * BLD + j;
+            W[0][o] += gr[vv];
+            W[1][o] += gi[vv];
+        }
+        __syncthreads();
+        emit(b, W[0], W[1]);
+        // rotate: G_b becomes the next G_{b+1}; the old Gn buffer is free
+        double* t0 = Gn[0];
+        double* t1 = Gn[1];
+        Gn[0] = W[0];
+        Gn[1] = W[1];
+        W[0] = t0;
+        W[1] = t1;
+    }
+}
+
+// out[j] = sum_e part[e][j] in energy order.
+__global__ void block_reduce_kernel(int nE, int len, const double* __restrict__ part, double* __restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    double a = 0.0;
+    for (int e = 0; e < nE; ++e) a += part[(size_t)e * len + j];
+    out[j] = a;
+}
+
+struct BlockChain {
+    int nb = 0;
+    std::vector<double> H0, T;
+    double Uc[2] = {0.0, 0.0};
+    DevBuf d_H0, d_T, d_v, d_E, d_coef, d_Sigma, d_iters, d_g, d_part, d_out, d_diag;
+    size_t scratch_limit = (size_t)40 << 30;  // bytes of g_b storage per launch
+    float ms_sr = 0.f, ms_rgf = 0.f;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    ~BlockChain()
+    {
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+
+}  // namespace negf
+
+using namespace negf;
+
+struct negf_block_chain {
+    BlockChain c;
+};
+
+extern "C" {
+
+int negf_block_chain_create(int m, int nblocks, const double* H0, const double* T, negf_block_chain** out)
+{
+    clear_error();
+    if (m != BM || nblocks < 1 || !H0 || !T || !out) {
+        set_error("block chain needs m = %d, nblocks >= 1 and H0, T", BM);
+        return NEGF_EARG;
+    }
+    for (int i = 0; i < BM; ++i)
+        for (int j = 0; j < BM; ++j)
+            if (H0[i * BM + j] != H0[j * BM + i]) {
+                set_error("H0 must be symmetric");
+                return NEGF_EARG;
+            }
+    negf_block_chain* h = new negf_block_chain;
+    BlockChain& c = h->c;
+    c.nb = nblocks;
+    c.H0.assign(H0, H0 + BM * BM);
+    c.T.assign(T, T + BM * BM);
+    int rc = 0;
+    auto fail = [&](int r) {
+        delete h;
+        return r;
+    };
+    if (hipStreamCreate(&c.st) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        return fail(NEGF_EDEVICE);
+    }
+    for (auto& x : c.ev)
+        if (hipEventCreate(&x) != hipSuccess) {
+            set_error("hipEventCreate failed");
+            return fail(NEGF_EDEVICE);
+        }
+    if ((rc = c.d_H0.ensure(sizeof(double) * BM * BM))) return fail(rc);
+    if ((rc = c.d_T.ensure(sizeof(double) * BM * BM))) return fail(rc);
+    if ((rc = c.d_v.ensure(sizeof(double) * (size_t)nblocks * BM))) return fail(rc);
+    if (hipMemcpy(c.d_H0.p, H0, sizeof(double) * BM * BM, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c.d_T.p, T, sizeof(double) * BM * BM, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c.d_v.p, 0, sizeof(double) * (size_t)nblocks * BM) != hipSuccess) {
+        set_error("block chain upload failed");
+        return fail(NEGF_EDEVICE);
+    }
+    *out = h;
+    return NEGF_OK;
+}
+
+int negf_block_chain_destroy(negf_block_chain* h)
+{
+    if (!h) return NEGF_EARG;
+    delete h;
+    return NEGF_OK;
+}
+
+int negf_block_chain_set_potential(negf_block_chain* h, const double* v, const double* Uc)
+{
+    clear_error();
+    if (!h || !v || !Uc) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    NEGF_HIP_TRY(hipMemcpy(h->c.d_v.p, v, sizeof(double) * (size_t)h->c.nb * BM, hipMemcpyHostToDevice));
+    h->c.Uc[0] = Uc[0];
+    h->c.Uc[1] = Uc[1];
+    return NEGF_OK;
+}
+
+int negf_block_rgf(negf_block_chain* h, int nE, const negf_complex* E, const negf_complex* coef, double sr_tol,
+                   int sr_max_iter, double* acc, negf_complex* diagG, int* sr_iters)
+{
+    clear_error();
+    if (!h || nE < 1 || !E || !coef || !(sr_tol > 0) || sr_max_iter < 1) {
+        set_error("invalid block RGF arguments");
+        return NEGF_EARG;
+    }
+    BlockChain& c = h->c;
+    const int nb = c.nb;
+    const size_t len = (size_t)nb * BM;
+    int rc;
+    if ((rc = c.d_E.ensure(sizeof(double2) * nE))) return rc;
+    if ((rc = c.d_coef.ensure(sizeof(double2) * nE))) return rc;
+    if ((rc = c.d_Sigma.ensure(sizeof(double) * (size_t)nE * 4 * BGL))) return rc;
+    if ((rc = c.d_iters.ensure(sizeof(int) * 2 * nE))) return rc;
+    if ((rc = c.d_part.ensure(sizeof(double) * (size_t)nE * len))) return rc;
+    if ((rc = c.d_out.ensure(sizeof(double) * len))) return rc;
+    if (diagG && (rc = c.d_diag.ensure(sizeof(double2) * (size_t)nE * len))) return rc;
+    const size_t per_energy = (size_t)std::max(nb - 1, 1) * 2 * BGL * sizeof(double);
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nE, c.scratch_limit / per_energy));
+    if ((rc = c.d_g.ensure(per_energy * chunk))) return rc;
+    NEGF_HIP_TRY(hipMemcpyAsync(c.d_E.p, E, sizeof(double2) * nE, hipMemcpyHostToDevice, c.st));
+    NEGF_HIP_TRY(hipMemcpyAsync(c.d_coef.p, coef, sizeof(double2) * nE, hipMemcpyHostToDevice, c.st));
+    NEGF_HIP_TRY(hipMemsetAsync(c.d_iters.p, 0, sizeof(int) * 2 * nE, c.st));
+    NEGF_HIP_TRY(hipEventRecord(c.ev[0], c.st));
+    hipLaunchKernelGGL(sancho_rubio_kernel, dim3(nE, 2), dim3(BTHREADS), 0, c.st, nE, c.d_E.as<double2>(),
+                       c.d_H0.as<double>(), c.d_T.as<double>(), c.Uc[0], c.Uc[1], sr_tol, sr_max_iter,
+                       c.d_Sigma.as<double>(), c.d_iters.as<int>());
+    NEGF_HIP_TRY(hipGetLastError());
+    NEGF_HIP_TRY(hipEventRecord(c.ev[1], c.st));
+    for (int e0 = 0; e0 < nE; e0 += chunk) {
+        const int n = std::min(chunk, nE - e0);
+        hipLaunchKernelGGL(block_rgf_kernel, dim3(n), dim3(BTHREADS), 0, c.st, e0, nE, nb, c.d_E.as<double2>(),
+                           c.d_coef.as<double2>(), c.d_H0.as<double>(), c.d_T.as<double>(), c.d_v.as<double>(),
+                           c.d_Sigma.as<double>(), c.d_g.as<double>(), c.d_part.as<double>(),
+                           diagG ? c.d_diag.as<double2>() : nullptr);
+        NEGF_HIP_TRY(hipGetLastError());
+    }
+    NEGF_HIP_TRY(hipEventRecord(c.ev[2], c.st));
+    hipLaunchKernelGGL(block_reduce_kernel, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, c.st, nE, (int)len,
+                       c.d_part.as<double>(), c.d_out.as<double>());
+    NEGF_HIP_TRY(hipGetLastError());
+    std::vector<int> it(2 * nE);
+    NEGF_HIP_TRY(hipMemcpyAsync(it.data(), c.d_iters.p, sizeof(int) * 2 * nE, hipMemcpyDeviceToHost, c.st));
+    if (acc) NEGF_HIP_TRY(hipMemcpyAsync(acc, c.d_out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c.st));
+    if (diagG)
+        NEGF_HIP_TRY(hipMemcpyAsync(diagG, c.d_diag.p, sizeof(double2) * (size_t)nE * len, hipMemcpyDeviceToHost,
+                                    c.st));
+    NEGF_HIP_TRY(hipStreamSynchronize(c.st));
+    NEGF_HIP_TRY(hipEventElapsedTime(&c.ms_sr, c.ev[0], c.ev[1]));
+    NEGF_HIP_TRY(hipEventElapsedTime(&c.ms_rgf, c.ev[1], c.ev[2]));
+    if (sr_iters) std::copy(it.begin(), it.end(), sr_iters);
+    for (int k = 0; k < 2 * nE; ++k)
+        if (it[k] <= 0) {
+            set_error("Sancho-Rubio did not reach the tolerance for energy %d within %d iterations", k / 2,
+                      sr_max_iter);
+            return NEGF_ENONCONV;
+        }
+    return NEGF_OK;
+}
+
+int negf_block_self_energies(negf_block_chain* h, int nE, const negf_complex* E, double sr_tol, int sr_max_iter,
+                              double* sigma, int* sr_iters)
+{
+    clear_error();
+    if (!h || nE < 1 || !E || !sigma || !(sr_tol > 0) || sr_max_iter < 1) {
+        set_error("invalid self-energy arguments");
+        return NEGF_EARG;
+    }
+    BlockChain& c = h->c;
+    int rc;
+    if ((rc = c.d_E.ensure(sizeof(double2) * nE))) return rc;
+    if ((rc = c.d_Sigma.ensure(sizeof(double) * (size_t)nE * 4 * BGL))) return rc;
+    if ((rc = c.d_iters.ensure(sizeof(int) * 2 * nE))) return rc;
+    NEGF_HIP_TRY(hipMemcpyAsync(c.d_E.p, E, sizeof(double2) * nE, hipMemcpyHostToDevice, c.st));
+    NEGF_HIP_TRY(hipMemsetAsync(c.d_iters.p, 0, sizeof(int) * 2 * nE, c.st));
+    hipLaunchKernelGGL(sancho_rubio_kernel, dim3(nE, 2), dim3(BTHREADS), 0, c.st, nE, c.d_E.as<double2>(),
+                       c.d_H0.as<double>(), c.d_T.as<double>(), c.Uc[0], c.Uc[1], sr_tol, sr_max_iter,
+                       c.d_Sigma.as<double>(), c.d_iters.as<int>());
+    NEGF_HIP_TRY(hipGetLastError());
+    std::vector<int> it(2 * nE);
+    NEGF_HIP_TRY(hipMemcpyAsync(it.data(), c.d_iters.p, sizeof(int) * 2 * nE, hipMemcpyDeviceToHost, c.st));
+    NEGF_HIP_TRY(hipMemcpyAsync(sigma, c.d_Sigma.p, sizeof(double) * (size_t)nE * 4 * BGL, hipMemcpyDeviceToHost,
+                                c.st));
+    NEGF_HIP_TRY(hipStreamSynchronize(c.st));
+    if (sr_iters) std::copy(it.begin(), it.end(), sr_iters);
+    for (int k = 0; k < 2 * nE; ++k)
+        if (it[k] <= 0) {
+            set_error("Sancho-Rubio did not reach the tolerance for energy %d within %d iterations", k / 2,
+                      sr_max_iter);
+            return NEGF_ENONCONV;
+        }
+    return NEGF_OK;
+}
+
+int negf_block_last_time(const negf_block_chain* h, float* ms_sancho_rubio, float* ms_rgf)
+{
+    if (!h) return NEGF_EARG;
+    if (ms_sancho_rubio) *ms_sancho_rubio = h->c.ms_sr;
+    if (ms_rgf) *ms_rgf = h->c.ms_rgf;
+    return NEGF_OK;
+}
+
+}  // extern "C"
+
+namespace negf {
+// op 0: C = A B (complex), op 1: C = A^-1 (complex), op 2: C = A B with
+// B real (B im plane ignored); A, B, C: [re plane, im plane] of 32 x 32.
+__global__ __launch_bounds__(BTHREADS) void block_probe_kernel(int op, const double* __restrict__ A,
+                                                               const double* __restrict__ Bm, double* __restrict__ Cm)
+{
+    __shared__ double a[2][BPL], b[2][BPL];
+    __shared__ unsigned long long keys[16];
+    __shared__ int perm[BM];
+    const Quad q;
+    load_cplx(a[0], a[1], A);
+    load_cplx(b[0], b[1], Bm);
+    __syncthreads();
+    if (op == 1) {
+        gj_inverse(a[0], a[1], b[0], b[1], keys, perm);
+        store_cplx(Cm, a[0], a[1]);
+        return;
+    }
+    d4 r = {0, 0, 0, 0}, i = {0, 0, 0, 0};
+    if (op == 0)
+        mm<true, true, false, false>(r, i, a[0], a[1], b[0], b[1], q);
+    else
+        mm<true, false, false, false>(r, i, a[0], a[1], b[0], nullptr, q);
+    __syncthreads();
+    store_quad(a[0], a[1], r, i, q);
+    __syncthreads();
+    store_cplx(Cm, a[0], a[1]);
+}
+}  // namespace negf
+
+extern "C" int negf_block_probe(int op, const double* A, const double* B, double* C)
+{
+    negf::clear_error();
+    if (op < 0 || op > 2 || !A || !B || !C) {
+        negf::set_error("invalid probe arguments");
+        return NEGF_EARG;
+    }
+    negf::DevBuf d;
+    if (d.ensure(sizeof(double) * 3 * 2 * BGL)) return NEGF_EALLOC;
+    double* p = d.as<double>();
+    NEGF_HIP_TRY(hipMemcpy(p, A, sizeof(double) * 2 * BGL, hipMemcpyHostToDevice));
+    NEGF_HIP_TRY(hipMemcpy(p + 2 * BGL, B, sizeof(double) * 2 * BGL, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(negf::block_probe_kernel, dim3(1), dim3(BTHREADS), 0, 0, op, p, p + 2 * BGL, p + 4 * BGL);
+    NEGF_HIP_TRY(hipGetLastError());
+    NEGF_HIP_TRY(hipDeviceSynchronize());
+    NEGF_HIP_TRY(hipMemcpy(C, p + 4 * BGL, sizeof(double) * 2 * BGL, hipMemcpyDeviceToHost));
+    return NEGF_OK;
+}

@@ -418,3 +418,68 @@ def conductance(E, T, mu0, kT0):
     tot = C.c_double()
     _lib.check(_lib.load().negf_conductance(E.size, _dp(E), _dp(T), mu0, kT0, _dp(cond), C.byref(tot)))
     return cond, tot.value
+
+
+class BlockChain:
+    """Chain of dense 32 x 32 blocks (negf_block_*): on-site H0 + diag(v_b),
+    coupling T = H_{b,b+1}, leads continuing the chain; Sancho-Rubio lead
+    surface Green's functions and the block RGF sweep on FP64 MFMA."""
+
+    M = 32
+
+    def __init__(self, nblocks, H0, T):
+        self.lib = _lib.load()
+        self.nb = nblocks
+        H0 = np.ascontiguousarray(H0, np.float64)
+        T = np.ascontiguousarray(T, np.float64)
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_block_chain_create(self.M, nblocks, _dp(H0), _dp(T), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.negf_block_chain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_potential(self, v, Uc=(0.0, 0.0)):
+        v = np.ascontiguousarray(v, np.float64).reshape(self.nb * self.M)
+        uc = np.ascontiguousarray(Uc, np.float64)
+        _lib.check(self.lib.negf_block_chain_set_potential(self.h, _dp(v), _dp(uc)))
+
+    def rgf(self, E, coef=None, tol=1e-12, max_iter=200, want_diag=False):
+        """(acc [nb, 32], diagG [nE, nb, 32] or None, Sancho-Rubio iterations [nE, 2])."""
+        E = np.ascontiguousarray(np.atleast_1d(E), np.complex128)
+        nE = E.size
+        coef = np.ones(nE, np.complex128) if coef is None else np.ascontiguousarray(coef, np.complex128)
+        acc = np.zeros(self.nb * self.M)
+        diag = np.zeros(nE * self.nb * self.M, np.complex128) if want_diag else None
+        it = np.zeros(2 * nE, np.int32)
+        _lib.check(self.lib.negf_block_rgf(self.h, nE, E.ctypes.data_as(C.c_void_p), coef.ctypes.data_as(C.c_void_p),
+                                           tol, max_iter, _dp(acc),
+                                           None if diag is None else diag.ctypes.data_as(C.c_void_p), _ip(it)))
+        return (acc.reshape(self.nb, self.M),
+                None if diag is None else diag.reshape(nE, self.nb, self.M), it.reshape(nE, 2))
+
+    def self_energies(self, E, tol=1e-12, max_iter=200):
+        """(Sigma_L, Sigma_R) [nE, 32, 32] complex and iterations [nE, 2]."""
+        E = np.ascontiguousarray(np.atleast_1d(E), np.complex128)
+        nE = E.size
+        sig = np.zeros(nE * 4 * self.M * self.M)
+        it = np.zeros(2 * nE, np.int32)
+        _lib.check(self.lib.negf_block_self_energies(self.h, nE, E.ctypes.data_as(C.c_void_p), tol, max_iter,
+                                                     _dp(sig), _ip(it)))
+        s = sig.reshape(nE, 2, 2, self.M, self.M)
+        z = s[:, :, 0] + 1j * s[:, :, 1]
+        return z[:, 0], z[:, 1], it.reshape(nE, 2)
+
+    def last_time(self):
+        a = C.c_float()
+        b = C.c_float()
+        _lib.check(self.lib.negf_block_last_time(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value

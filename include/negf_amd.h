@@ -319,6 +319,43 @@ int negf_batch_scf_run(negf_batch_scf *s, const double *V_gate, double *U_out,
                        int *total_iterations);
 
 /* ------------------------------------------------------------------------
+ * Dense-block chains on FP64 MFMA.  The reference declares this path but
+ * leaves it stubbed: the 2-D MatrixBlock multiply/inverse/dagger are empty
+ * (Matrix_Block_Util.H:123-135, 431-438, 684-692) and DecimationTechnique
+ * (NEGF_Common.cpp:4095-4153) works only for 1-D blocks.  A chain of
+ * nblocks blocks of m = 32 orbitals has on-site H0 + diag(v_b) (H0 real
+ * symmetric m x m, row-major; v_b real, per orbital) and the same real
+ * coupling T = H_{b,b+1} between consecutive blocks; the leads continue the
+ * chain with on-site H0 + Uc[0] (left) and H0 + Uc[1] (right).
+ * ------------------------------------------------------------------------ */
+typedef struct negf_block_chain negf_block_chain;
+int negf_block_chain_create(int m, int nblocks, const double *H0, const double *T,
+                            negf_block_chain **out);
+int negf_block_chain_destroy(negf_block_chain *c);
+/* v: [nblocks][m] orbital potentials (eV); Uc[2]: lead potentials. */
+int negf_block_chain_set_potential(negf_block_chain *c, const double *v, const double *Uc);
+/* For every energy E[e]: lead surface Green's functions by Sancho-Rubio
+ * decimation (until max|alpha|, |beta| < sr_tol, at most sr_max_iter
+ * iterations), Sigma_L = T^T g_L T, Sigma_R = T g_R T^T, then the block RGF
+ * sweep for diag G^R.  acc[b*m + o] = sum_e Im(coef[e] G_e,bb,oo) (the block
+ * form of the charge accumulation of NEGF_Common.cpp:3742-3754);
+ * diagG [nE][nblocks*m] (may be NULL); sr_iters [nE][2] (may be NULL).
+ * NEGF_ENONCONV if some decimation did not reach sr_tol. */
+int negf_block_rgf(negf_block_chain *c, int nE, const negf_complex *E,
+                   const negf_complex *coef, double sr_tol, int sr_max_iter,
+                   double *acc, negf_complex *diagG, int *sr_iters);
+/* The lead self-energies alone: sigma [nE][2 leads][re, im planes][m*m]
+ * (row-major), sr_iters [nE][2] (may be NULL). */
+int negf_block_self_energies(negf_block_chain *c, int nE, const negf_complex *E,
+                             double sr_tol, int sr_max_iter, double *sigma, int *sr_iters);
+/* Self-check of the device building blocks on one 32 x 32 complex matrix
+ * pair (re plane then im plane, row-major): op 0: C = A B, op 1: C = A^-1
+ * (pivoted Gauss-Jordan), op 2: C = A Re(B).  No reference counterpart. */
+int negf_block_probe(int op, const double *A, const double *B, double *C);
+/* Device time (ms) of the decimation and of the RGF sweep of the last call. */
+int negf_block_last_time(const negf_block_chain *c, float *ms_sancho_rubio, float *ms_rgf);
+
+/* ------------------------------------------------------------------------
  * Charge surrogate (no reference counterpart; a new component next to the
  * NEGF charge path).  A 1D-convolutional network maps a tube's potential
  * U (eV, N sites) and one per-sample feature f (e.g. the drain bias) to an

@@ -1,0 +1,124 @@
+"""GPU parity of the dense-block path (negf_block_*: Sancho-Rubio leads and
+block RGF on FP64 MFMA) against oracle/block_oracle.py on the same inputs.
+
+Tolerances: diag G^R and the accumulated charge 1e-10 relative (max |d| /
+max |ref|); the GPU's pivoted Gauss-Jordan and MFMA sums round differently
+from numpy's LAPACK inverse and matmul.
+"""
+import numpy as np
+import pytest
+
+from electrostatic_amd import negf
+from oracle import block_oracle as B
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def chain(nb, seed, sigma=0.05, Uc=(0.1, -0.05)):
+    H0, T = B.zigzag_blocks(16, 2.5)
+    rng = np.random.default_rng(seed)
+    smooth = 0.2 * np.sin(np.linspace(0, np.pi, nb))[:, None]
+    v = rng.normal(0.0, sigma, (nb, 32)) + smooth
+    return H0, T, v, Uc
+
+
+def test_sancho_rubio_self_energies_match_oracle(dev):
+    H0, T = B.zigzag_blocks(16, 2.5)
+    c = negf.BlockChain(3, H0, T)
+    Uc = (0.1, -0.2)
+    c.set_potential(np.zeros((3, 32)), Uc)
+    E = np.array([0.37 + 1e-5j, -0.8 + 1e-5j, 1.9 + 1e-3j, -2.6 + 0.05j, 0.02 + 1e-5j, 5.5 + 1e-5j])
+    SL, SR, it = c.self_energies(E)
+    for k, z in enumerate(E):
+        rl, rr, its = B.lead_self_energies(z, H0, T, Uc)
+        assert rel(SL[k], rl) < 1e-10, (k, rel(SL[k], rl))
+        assert rel(SR[k], rr) < 1e-10, (k, rel(SR[k], rr))
+        assert abs(it[k, 0] - its[0]) <= 1 and abs(it[k, 1] - its[1]) <= 1
+    c.close()
+
+
+@pytest.mark.parametrize("nb", [1, 2, 7])
+def test_block_rgf_diag_matches_oracle(dev, nb):
+    H0, T, v, Uc = chain(nb, seed=nb)
+    E = np.array([0.37 + 1e-5j, -0.8 + 1e-5j, 1.9 + 1e-3j, -2.6 + 0.05j, 0.02 + 1e-5j])
+    c = negf.BlockChain(nb, H0, T)
+    c.set_potential(v, Uc)
+    acc, dg, it = c.rgf(E, want_diag=True)
+    assert np.all(it > 0)
+    ref_acc = np.zeros((nb, 32))
+    for k, z in enumerate(E):
+        SL, SR, its = B.lead_self_energies(z, H0, T, Uc)
+        ref = B.block_rgf_diag(z, H0, T, v, SL, SR)
+        assert rel(dg[k], ref) < 1e-10, (k, rel(dg[k], ref))
+        assert abs(it[k, 0] - its[0]) <= 1 and abs(it[k, 1] - its[1]) <= 1
+        ref_acc += np.imag(ref)
+    assert rel(acc, ref_acc) < 1e-10
+    c.close()
+
+
+def test_block_rgf_long_chain_charge_and_coefficients(dev):
+    nb = 300
+    H0, T, v, Uc = chain(nb, seed=11)
+    rng = np.random.default_rng(12)
+    E = rng.uniform(-3, 3, 6) + 1j * rng.uniform(1e-5, 1e-2, 6)
+    coef = rng.standard_normal(6) + 1j * rng.standard_normal(6)
+    c = negf.BlockChain(nb, H0, T)
+    c.set_potential(v, Uc)
+    acc, _, it = c.rgf(E, coef)
+    ref = B.accumulate(E, coef, H0, T, v, Uc)
+    assert rel(acc, ref) < 1e-10
+    c.close()
+
+
+def test_block_rgf_ring_traces_match_mode_space(dev):
+    # a ring-uniform potential and no disorder: ring traces of the block
+    # path equal the sum over angular modes of scalar chains (oracle)
+    nb, m, gamma = 6, 16, 2.5
+    H0, T = B.zigzag_blocks(m, gamma)
+    Uring = np.linspace(-0.2, 0.25, 2 * nb)
+    v = np.repeat(Uring.reshape(nb, 2), m, axis=1)
+    c = negf.BlockChain(nb, H0, T)
+    c.set_potential(v, (Uring[0], Uring[-1]))
+    E = np.array([0.41 + 1e-3j, -1.3 + 1e-3j])
+    _, dg, _ = c.rgf(E, want_diag=True)
+    from tests.test_oracle import _mode_chain_diag
+    for k, z in enumerate(E):
+        ring = dg[k].reshape(nb, 2, m).sum(axis=2).ravel()
+        modes = sum(_mode_chain_diag(z, 2 * gamma * abs(np.cos(np.pi * J / m)), gamma, Uring,
+                                     (Uring[0], Uring[-1])) for J in range(m))
+        assert rel(ring, modes) < 1e-10
+    c.close()
+
+
+def test_block_rgf_rejects_bad_arguments(dev):
+    H0, T = B.zigzag_blocks(16)
+    with pytest.raises(Exception):
+        negf.BlockChain(4, H0 + np.triu(np.ones((32, 32)), 1), T)   # not symmetric
+    c = negf.BlockChain(2, H0, T)
+    with pytest.raises(Exception):
+        c.rgf(np.array([0.1 + 1e-5j]), tol=-1.0)
+    c.close()
+
+
+@pytest.mark.parametrize("op", [0, 1, 2])
+def test_block_building_blocks(dev, op):
+    # the MFMA quadrant products (operand and result lane layout) and the
+    # pivoted Gauss-Jordan inverse on one matrix pair, against numpy
+    import ctypes as C
+    from electrostatic_amd import _lib
+    rng = np.random.default_rng(op)
+    A = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    A[:, 3] *= 1e-3                     # forces row exchanges in the inverse
+    Bm = rng.standard_normal((32, 32)) + 1j * rng.standard_normal((32, 32))
+    planes = lambda M: np.ascontiguousarray(np.stack([M.real, M.imag]).ravel())
+    a, b = planes(A), planes(Bm)
+    out = np.zeros(2048)
+    _lib.check(_lib.load().negf_block_probe(op, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
+                                            out.ctypes.data_as(C.c_void_p)))
+    Cm = out[:1024].reshape(32, 32) + 1j * out[1024:].reshape(32, 32)
+    ref = {0: A @ Bm, 1: np.linalg.inv(A), 2: A @ Bm.real}[op]
+    assert rel(Cm, ref) < 1e-12

@@ -458,3 +458,61 @@ def test_bf16_rounding_is_nearest_even():
     assert S.bf16(one + 1.5 * ulp) == one + 2 * ulp  # tie -> even
     assert S.bf16(one + 0.51 * ulp) == one + ulp
     assert S.bf16(-3.0) == -3.0
+
+
+# ---------------------------------------------------------------- dense-block path
+def _mode_chain_diag(E, beta, gamma, U, Uc):
+    """Scalar mode-space chain (hoppings beta on even bonds, gamma on odd
+    bonds) with the reference's closed-form lead self-energies."""
+    N = len(U)
+    H = np.diag(np.asarray(U, np.complex128))
+    for n in range(N - 1):
+        t = beta if n % 2 == 0 else gamma
+        H[n, n + 1] = H[n + 1, n] = -t
+    H[0, 0] += gamma ** 2 * O.surface_gf(E, Uc[0], beta, gamma)[0]
+    H[-1, -1] += gamma ** 2 * O.surface_gf(E, Uc[1], beta, gamma)[0]
+    return np.diag(np.linalg.inv(E * np.eye(N) - H))
+
+
+@pytest.mark.parametrize("E", [0.31 + 1e-3j, -1.7 + 1e-3j, 0.05 + 1e-3j, 2.9 + 1e-3j])
+def test_block_sancho_rubio_matches_closed_form_per_mode(E):
+    from oracle import block_oracle as B
+    m, gamma = 16, 2.5
+    H0, T = B.zigzag_blocks(m, gamma)
+    gR, it = B.sancho_rubio(E - 0.1, H0, T, T.T)
+    assert it < 200
+    j = np.arange(m)
+    F = np.exp(2j * np.pi * np.outer(j, j) / m) / np.sqrt(m)
+    gm = F.conj().T @ gR[:m, :m] @ F
+    ref = np.array([O.surface_gf(E, 0.1, 2 * gamma * abs(np.cos(np.pi * J / m)), gamma)[0] for J in range(m)])
+    assert np.abs(np.diag(gm) - ref).max() < 1e-10 * np.abs(ref).max()
+    assert np.abs(gm - np.diag(np.diag(gm))).max() < 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("E", [0.31 + 1e-3j, -1.2 + 1e-3j])
+def test_block_rgf_ring_traces_match_mode_space_chains(E):
+    from oracle import block_oracle as B
+    m, gamma, nb = 16, 2.5, 4
+    H0, T = B.zigzag_blocks(m, gamma)
+    rng = np.random.default_rng(3)
+    Uring = rng.uniform(-0.3, 0.3, 2 * nb)
+    v = np.repeat(Uring.reshape(nb, 2), m, axis=1)
+    Uc = (Uring[0], Uring[-1])
+    SL, SR, _ = B.lead_self_energies(E, H0, T, Uc)
+    Gb = B.block_rgf_diag(E, H0, T, v, SL, SR)
+    ring = Gb.reshape(nb, 2, m).sum(axis=2).ravel()
+    modes = sum(_mode_chain_diag(E, 2 * gamma * abs(np.cos(np.pi * J / m)), gamma, Uring, Uc) for J in range(m))
+    assert np.abs(ring - modes).max() < 1e-10 * np.abs(modes).max()
+
+
+def test_block_rgf_matches_dense_inverse_with_disorder():
+    from oracle import block_oracle as B
+    m, nb = 16, 5
+    H0, T = B.zigzag_blocks(m)
+    rng = np.random.default_rng(4)
+    v = rng.normal(0.0, 0.05, (nb, 2 * m)) + 0.1
+    for E in (0.4 + 1e-5j, -0.9 + 1e-5j, 0.2 + 0.3j):
+        SL, SR, _ = B.lead_self_energies(E, H0, T, (0.1, 0.1))
+        a = B.block_rgf_diag(E, H0, T, v, SL, SR)
+        b = B.dense_diag(E, H0, T, v, SL, SR)
+        assert np.abs(a - b).max() < 1e-9 * np.abs(b).max()
