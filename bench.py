@@ -157,6 +157,84 @@ def batch_leg(dev, steps=10):
     return out
 
 
+def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
+    """128 (17,0) tubes of N = 2048 sites (512 unit cells, 1 mode), contacts
+    at -1 eV, contour 30/30/64 + poles, dense 1/r-with-image coupling
+    (eps_r 3.9, R 0.665 nm, image 10 nm), U_ext = 4 seeded Gaussians
+    (amplitude 0.3 eV, seed 100 + t), gates V_t = linspace(0, 1, 128) V;
+    convergence at max |n_in - n_out| <= 1e-8.  The BF16 surrogate is trained
+    first (not timed) on n_pairs exact NEGF (U, Q) pairs of seeded smooth
+    potentials.  Timed: one batched run from n = 0, plain Broyden (fraction
+    0.1) against the surrogate-seeded, preconditioned run."""
+    import torch
+    from electrostatic_amd import negf
+    from electrostatic_amd.surrogate import Surrogate, negf_pairs, sample_potentials
+    from electrostatic_amd.workloads import gaussian_bumps
+    T, cells = 128, 512
+    tubes = [negf.Tube(num_unitcells=cells, num_modes=1, contact_mu=(-1.0, -1.0),
+                       eq_integration_pts=(30, 30, 64), noneq_integration_pts=(200,),
+                       E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0) for _ in range(T)]
+    N = tubes[0].N
+    batch = negf.Batch(tubes)
+    t0 = time.perf_counter()
+    dU, dQ = negf_pairs(batch, sample_potentials(n_pairs, N, seed=5), device=dev)
+    gen_s = time.perf_counter() - t0
+    sg = Surrogate(N, seed=1)
+    t0 = time.perf_counter()
+    loss = sg.train(dU, dQ, epochs=epochs, batch=32, lr=2e-3, lr_end=1e-4, seed=3)
+    train_s = time.perf_counter() - t0
+    del dU, dQ
+    torch.cuda.empty_cache()
+    sp = dict(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+    scf = negf.BatchScf(batch, Broyden_fraction=0.1, Broyden_max_norm=1e-8, Broyden_threshold_maxstep=300, **sp)
+    U_ext = np.stack([gaussian_bumps(N, seed=100 + t, amp=0.3) for t in range(T)])
+    V = np.linspace(0.0, 1.0, T)
+    scf.set_external_potential(U_ext)
+    scf.set_surrogate(sg)
+    out = {"workload": "128 (17,0) tubes x 2048 sites, 1 mode, dense 1/r coupling, gates 0..1 V, norm <= 1e-8",
+           "pairs": n_pairs, "epochs": epochs, "pair_generation_s": gen_s, "training_s": train_s,
+           "train_loss": loss}
+    runs = {}
+    variants = (("plain", lambda: scf.run(V)),
+                ("surrogate", lambda: scf.run_surrogate(V, seed_iters=6, refresh_iter=0)))
+    for key, fn in variants:
+        scf.set_input_charge(np.zeros((T, N)))
+        fn()                      # warm-up: graph capture, path set-up
+        scf.set_input_charge(np.zeros((T, N)))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        dt = time.perf_counter() - t0
+        runs[key] = r
+        out[key] = {"s": dt, "bias_points_per_s": T / dt, "iterations_max": int(r[2].max()),
+                    "iterations_mean": float(r[2].mean()), "batched_iterations": int(r[4])}
+    U0 = runs["plain"][0]
+    for key in ("surrogate",):
+        out[key]["max_abs_dU_vs_plain_V"] = float(np.abs(runs[key][0] - U0).max())
+        out[key]["speedup_vs_plain"] = out["plain"]["s"] / out[key]["s"]
+    out["capacitance_e_per_eV"] = [float(runs["surrogate"][5].min()), float(runs["surrogate"][5].max())]
+    scf.close()
+    sg.close()
+    batch.close()
+    if with_cpu:
+        from oracle import oracle as O
+        _, betas, degs = tubes[0].modes()
+        K = O.coulomb_kernel(tubes[0].positions(), **sp)
+        t0 = time.perf_counter()
+        for t in (0, T - 1):
+            ot = O.make_tube(N, list(betas), list(degs), mu=(-1.0, -1.0), eq_pts=(30, 30, 64),
+                             noneq_pts=(200,), eta=1e-5, E_valence_min=-10.0, E_pole_max=3.0)
+            r0, _ = O.rho0(ot)
+            ob = O.Broyden(N, 0.1, 300, 0)
+            O.scf_bias_point(ot, r0, K, V[t], ob, max_norm=1e-8, max_step=300, U_ext=U_ext[t])
+        dt = time.perf_counter() - t0
+        out["cpu_bias_points_per_s"] = 2 / dt
+        out["cpu_kind"] = "port (oracle serial SCF, reference loop order), 1 thread, 2 tubes"
+    for t in tubes:
+        t.close()
+    return out
+
+
 def block_leg(steps=3, with_cpu=True):
     """Dense-block chain: atomistic (16,0) tube in blocks of two rings (32
     orbitals), 2000 blocks, 1024 energies (512 on a contour arc, 512 on the
@@ -217,6 +295,8 @@ def main():
                     help="RGF segment length (0: library default)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the secondary 64-tube batched-solve measurement")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate-preconditioned 128-tube SCF leg")
     ap.add_argument("--no-block", action="store_true",
                     help="skip the dense-block (32-orbital) chain leg")
     ap.add_argument("--no-scf", action="store_true",
@@ -345,6 +425,8 @@ def main():
         out["batch"] = batch_leg(dev)
     if not args.no_block:
         out["block"] = block_leg(with_cpu=not args.no_cpu_baseline)
+    if not args.no_surrogate:
+        out["surrogate_scf"] = surrogate_leg(dev, with_cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)
 
 

@@ -136,6 +136,10 @@ SIGNATURES = {
     "negf_block_self_energies": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_int, dp, ip]),
     "negf_block_probe": (C.c_int, [C.c_int, vp, vp, vp]),
     "negf_block_last_time": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "negf_batch_scf_get_kernel": (C.c_int, [vp, dp]),
+    "negf_batch_scf_set_preconditioner": (C.c_int, [vp, dp, dp]),
+    "negf_batch_scf_set_surrogate": (C.c_int, [vp, vp, dp]),
+    "negf_batch_scf_run_surrogate": (C.c_int, [vp, dp, C.c_int, C.c_int, dp, dp, ip, dp, ip, dp]),
     "negf_surrogate_create": (C.c_int, [C.c_int, C.c_ulonglong, C.POINTER(vp)]),
     "negf_surrogate_destroy": (C.c_int, [vp]),
     "negf_surrogate_num_params": (C.c_int, [vp]),

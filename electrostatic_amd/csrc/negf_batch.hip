@@ -90,12 +90,14 @@ class Batch {
             RgfTube r = t.rgf_tube(t.d_U.as<double>(), t.d_Uc.as<double>(), (int)t.eq_host.size(), seg_len);
             r.chain0 = (int)eq.size();
             r.out_off = off[i];
+            r.ext = i;
             eq.insert(eq.end(), t.eq_host.begin(), t.eq_host.end());
             teq.push_back(r);
             if (!t.neq_host.empty()) {
                 RgfTube q = t.rgf_tube(t.d_U.as<double>(), t.d_Uc.as<double>(), (int)t.neq_host.size(), seg_len);
                 q.chain0 = (int)neq.size();
                 q.out_off = off[i];
+                q.ext = i;
                 for (int w = 0; w < 3; ++w) q.sample[w] = -1;  // no integrand in the batched launch
                 neq.insert(neq.end(), t.neq_host.begin(), t.neq_host.end());
                 tneq.push_back(q);
@@ -125,11 +127,11 @@ class Batch {
     }
 
     int enqueue(double* dQ) { return enqueue_on(dQ, stream); }
-    int enqueue_on(double* dQ, hipStream_t st)
+    int enqueue_on(double* dQ, hipStream_t st, const int* tube_active = nullptr)
     {
         int rc;
-        if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st))) return rc;
-        if (n_neq > 0 && (rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, st))) return rc;
+        if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st, tube_active))) return rc;
+        if (n_neq > 0 && (rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, st, tube_active))) return rc;
         const long long Nt = off.back();
         hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, st, Nt,
                            d_eq.as<double>(), d_neq.as<double>(), d_rho0.as<double>(), dQ);
@@ -300,7 +302,10 @@ struct negf_batch {
 
 namespace negf {
 int batch_prepare(negf_batch* b, int iter, bool* rebuilt) { return b->b.prepare(iter, rebuilt); }
-int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st) { return b->b.enqueue_on(dQ, st); }
+int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st, const int* tube_active)
+{
+    return b->b.enqueue_on(dQ, st, tube_active);
+}
 }  // namespace negf
 
 extern "C" {

@@ -215,6 +215,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         w.item0 = items;
         w.part_off = poff;
         for (int q = 0; q < 3; ++q) w.sample[q] = tb.sample[q];
+        w.ext = tb.ext;
         const int G = (tb.nchain + 63) / 64;
         for (int i = 0; i < G * P; ++i) {
             item_tube.push_back(t);
@@ -286,7 +287,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     return NEGF_OK;
 }
 
-int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st)
+int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st, const int* tube_active)
 {
     if (nitems_ == 0) {
         set_error("RgfPlan::enqueue: plan not built");
@@ -321,6 +322,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     lp.sblk_chain = d_sblk_chain_.as<int>();
     lp.status = d_status_.as<int>();
     lp.kind = kind_;
+    lp.tube_active = tube_active;
 
     hipLaunchKernelGGL(sigma_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, d_chains_,
                        d_chain_tube_.as<int>(), nchains_, d_tubes_.as<WTube>(), dec_.use, dec_.layers,
@@ -378,7 +380,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     dim3 g1((max_N_ + 15) / 16, ntubes_);
     hipLaunchKernelGGL(reduce_rows1_kernel, g1, dim3(256), 0, st, d_part_.as<double>(),
                        d_red_src_off_.as<long long>(), d_red_nrows_.as<int>(), d_red_N_.as<int>(), d_out,
-                       d_red2_dst_off_.as<long long>());
+                       d_red2_dst_off_.as<long long>(), d_tubes_.as<WTube>(), tube_active);
     NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }

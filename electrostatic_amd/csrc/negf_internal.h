@@ -67,6 +67,7 @@ struct RgfTube {
     int L = 0;                    // requested segment length (0: default)
     int chain0 = 0, nchain = 0;   // chains [chain0, chain0+nchain) of d_chains
     long long out_off = 0;        // offset of this tube's N outputs in d_out
+    int ext = 0;                  // index of the tube for an active mask
 };
 
 // A reusable plan: tables and workspace sized once, enqueued many times.
@@ -81,7 +82,10 @@ class RgfPlan {
     // the fixed-order row reduction.  d_out: per-tube outputs at out_off
     // (KIND_EQ/NONEQ); d_diagG: [chain][N]; d_chain_out: per-chain outputs
     // (integrand samples [slot][3], DOS/T pairs [slot][2]).
-    int enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st);
+    // tube_active (device, indexed by RgfTube::ext, may be null): tubes with 0
+    // are skipped and keep their previous outputs
+    int enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st,
+                const int* tube_active = nullptr);
     // Read and clear the device status word (synchronises the stream).
     int check_status(hipStream_t st, int* status);
     int num_items() const { return nitems_; }
@@ -122,7 +126,7 @@ int choose_segments(int N, int L_req, int* P, int* L);
 // solve (rebuilding plans when paths changed) and the bare launch sequence
 // on a given stream (capturable).
 int batch_prepare(negf_batch* b, int iter, bool* rebuilt);
-int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st);
+int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st, const int* tube_active = nullptr);
 int max_sites();
 
 }  // namespace negf

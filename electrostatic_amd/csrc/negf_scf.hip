@@ -18,6 +18,7 @@
 
 #include "negf_internal.h"
 #include "negf_material.h"
+#include "surrogate.h"
 
 namespace negf {
 
@@ -35,6 +36,7 @@ struct BroydenDev {
     double* W;          // [max_step + 1][S]
     double* intermed;   // [max_step + 1]
     double* out;        // [0] max norm, [1] l2 norm, [2] |dF|^2
+    double* PFp;        // previous preconditioned residual (preconditioned update only)
 };
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -66,8 +68,12 @@ __device__ void history_dots(const BroydenDev& b, int S, int m_hi, const double*
 }
 
 // One Broyden update of one system by one workgroup of kBroydenThreads.
+// PC: the mixing term fraction * F is replaced by a preconditioned residual
+// PF = P F (P linear and fixed between history resets), and fraction * dF by
+// PF - PF_previous; everything else is the reference update.
+template <bool PC>
 __device__ void broyden_update(int S, int m, double fraction, int norm_type, const double* __restrict__ n_out,
-                               const BroydenDev& b)
+                               const double* __restrict__ PF, const BroydenDev& b)
 {
     __shared__ double red[3][kWaves];
     __shared__ double hist[512];
@@ -124,8 +130,9 @@ __device__ void broyden_update(int S, int m, double fraction, int norm_type, con
             double sv = 0.0;
             for (int it = 1; it <= m - 1; ++it) sv = fma(b.W[(size_t)it * S + s], it < 512 ? hist[it] : b.intermed[it], sv);
             const double d = b.dF[s];
+            const double pd = PC ? PF[s] - b.PFp[s] : fraction * d;
             b.V[(size_t)m * S + s] = d / denom;
-            b.W[(size_t)m * S + s] = -fraction * d + (b.n_in[s] - b.n_prev[s]) - sv;
+            b.W[(size_t)m * S + s] = -pd + (b.n_in[s] - b.n_prev[s]) - sv;
         }
         __syncthreads();
         // sum_vec = W^T (V F) including this update (:137-170)
@@ -143,8 +150,10 @@ __device__ void broyden_update(int S, int m, double fraction, int norm_type, con
     for (int s = tid; s < S; s += kBroydenThreads) {
         const double prev = b.n_in[s];
         const double sv = m > 0 ? b.sum_vec[s] : 0.0;
+        const double pf = PC ? PF[s] : fraction * b.F[s];
+        if (PC) b.PFp[s] = PF[s];
         b.n_prev[s] = prev;
-        b.n_in[s] = prev - fraction * b.F[s] - sv;
+        b.n_in[s] = prev - pf - sv;
     }
 }
 
@@ -152,7 +161,7 @@ __global__ __launch_bounds__(kBroydenThreads) void broyden_step_kernel(int S, in
                                                                          const double* __restrict__ n_out,
                                                                          BroydenDev b)
 {
-    broyden_update(S, m, fraction, norm_type, n_out, b);
+    broyden_update<false>(S, m, fraction, norm_type, n_out, nullptr, b);
 }
 
 // U_i = U_ext_i - V_gate - sum_j K_ij n_j: one wavefront per row, lanes
@@ -209,6 +218,7 @@ class Broyden {
         b.W = h + (size_t)(max_step + 1) * S;
         b.intermed = h + 2 * (size_t)(max_step + 1) * S;
         b.out = d_out.as<double>();
+        b.PFp = nullptr;
         return b;
     }
     // Reset_Broyden_Parallel (Broyden_Parallel_General.cpp:250-355).
@@ -293,56 +303,102 @@ static void coulomb_kernel(const std::vector<double>& ptd, const negf_scf_params
 // kernel K (N x N): every iteration is a GEMM U_t = U_ext,t - V_t - K n_t for
 // all tubes, one batched NEGF solve, and one Broyden workgroup per tube.
 
-// Y[t][i] = sum_j K[i][j] X[t][j] (K row-major N x N, X and Y T x N):
-// 32 x 32 output tiles, 2 x 2 per thread, 16-wide reduction chunks in LDS.
-__global__ __launch_bounds__(256) void dgemm_kx_kernel(int N, int T, const double* __restrict__ K,
-                                                       const double* __restrict__ X, size_t ldx,
-                                                       double* __restrict__ Y)
+// Partial products Yp[z][t][i] = sum_{j in chunk z} A[i][j] X[t][j]
+// (A row-major N x N; X rows ldx apart; T rows) on v_mfma_f64_16x16x4f64,
+// chunk z = [z kchunk, (z + 1) kchunk).  A workgroup of 4 wavefronts computes
+// a 32 (i) x 32 (t) tile, one 16 x 16 quadrant per wavefront, over 32-wide
+// j blocks staged in LDS; the next block's global loads are issued before
+// the present block's MFMAs.  Lane l supplies A[i0 + l % 16][j + l / 16] and
+// X[t0 + l % 16][j + l / 16]; result register v of lane l is the entry
+// (i = l / 16 + 4 v, t = l % 16).  The caller sums the chunks in order.
+typedef __attribute__((ext_vector_type(4))) double d4;
+__global__ __launch_bounds__(256) void mfma_dgemm_kernel(int N, int T, const double* __restrict__ A,
+                                                         const double* __restrict__ X, size_t ldx, int kchunk,
+                                                         double* __restrict__ Yp)
 {
-    __shared__ double Ks[32][17];
-    __shared__ double Xs[32][17];
+    __shared__ double As[32][33];
+    __shared__ double Xs[32][33];
     const int i0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
-    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads
-    double acc[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
-    for (int j0 = 0; j0 < N; j0 += 16) {
-        // 32 x 16 tiles of K (rows i) and X (rows t)
-        for (int q = threadIdx.x; q < 512; q += 256) {
-            const int r = q >> 4, c = q & 15;
+    const int jb = blockIdx.z * kchunk, je = min(N, jb + kchunk);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int qi = w >> 1, qt = w & 1;
+    double ra[4], rx[4];
+    auto fetch = [&](int j0) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            const int r = e >> 5, c = e & 31;
             const int i = i0 + r, t = t0 + r, j = j0 + c;
-            Ks[r][c] = (i < N && j < N) ? K[(size_t)i * N + j] : 0.0;
-            Xs[r][c] = (t < T && j < N) ? X[(size_t)t * ldx + j] : 0.0;
+            ra[q] = (i < N && j < je) ? A[(size_t)i * N + j] : 0.0;
+            rx[q] = (t < T && j < je) ? X[(size_t)t * ldx + j] : 0.0;
+        }
+    };
+    d4 acc = {0.0, 0.0, 0.0, 0.0};
+    fetch(jb);
+    for (int j0 = jb; j0 < je; j0 += 32) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = tid + 256 * q;
+            As[e >> 5][e & 31] = ra[q];
+            Xs[e >> 5][e & 31] = rx[q];
         }
         __syncthreads();
+        if (j0 + 32 < je) fetch(j0 + 32);
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const double k0 = Ks[ty][c], k1 = Ks[ty + 16][c];
-            const double x0 = Xs[tx][c], x1 = Xs[tx + 16][c];
-            acc[0][0] = fma(k0, x0, acc[0][0]);
-            acc[0][1] = fma(k0, x1, acc[0][1]);
-            acc[1][0] = fma(k1, x0, acc[1][0]);
-            acc[1][1] = fma(k1, x1, acc[1][1]);
+        for (int k0 = 0; k0 < 32; k0 += 4) {
+            const double a = As[qi * 16 + (lane & 15)][k0 + (lane >> 4)];
+            const double x = Xs[qt * 16 + (lane & 15)][k0 + (lane >> 4)];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, x, acc, 0, 0, 0);
         }
         __syncthreads();
     }
+    double* Y = Yp + (size_t)blockIdx.z * T * N;
+    const int t = t0 + qt * 16 + (lane & 15);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < 2; ++b) {
-            const int i = i0 + ty + 16 * a, t = t0 + tx + 16 * b;
-            if (i < N && t < T) Y[(size_t)t * N + i] = acc[a][b];
-        }
+    for (int v = 0; v < 4; ++v) {
+        const int i = i0 + qi * 16 + (lane >> 4) + 4 * v;
+        if (i < N && t < T) Y[(size_t)t * N + i] = acc[v];
+    }
+}
+
+// K-split of the GEMM: enough workgroups to cover the chip (about 1024),
+// chunks a multiple of 32 wide
+static int gemm_splits(int N, int T)
+{
+    const int tiles = ((N + 31) / 32) * ((T + 31) / 32);
+    int z = std::max(1, std::min(8, 1024 / std::max(tiles, 1)));
+    while (z > 1 && (N + 32 * z - 1) / (32 * z) < 4) --z;  // keep >= 4 blocks per chunk
+    return z;
+}
+
+static int gemm_chunk(int N, int z) { return ((N + z - 1) / z + 31) / 32 * 32; }
+
+static void launch_dgemm(int N, int T, int z, const double* A, const double* X, size_t ldx, double* Yp,
+                         hipStream_t st)
+{
+    dim3 g((N + 31) / 32, (T + 31) / 32, z);
+    hipLaunchKernelGGL(mfma_dgemm_kernel, g, dim3(256), 0, st, N, T, A, X, ldx, gemm_chunk(N, z), Yp);
+}
+
+// Y = sum of the z partial products, in chunk order
+__device__ __forceinline__ double sum_parts(const double* __restrict__ P, int z, long long NT, long long q)
+{
+    double a = P[q];
+    for (int k = 1; k < z; ++k) a += P[k * NT + q];
+    return a;
 }
 
 // U_t = U_ext,t - V_t - KN_t, scattered into each tube's potential and
 // contact potentials; also kept in Uall for the caller.
 __global__ void batch_potential_kernel(int N, int T, const double* __restrict__ Uext, const double* __restrict__ V,
-                                       const double* __restrict__ KN, double* const* __restrict__ dU,
+                                       const double* __restrict__ KN, int nz, double* const* __restrict__ dU,
                                        double* const* __restrict__ dUc, double* __restrict__ Uall)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)N * T) return;
+    const long long NT = (long long)N * T;
+    if (q >= NT) return;
     const int t = (int)(q / N), i = (int)(q - (long long)t * N);
-    const double u = (Uext ? Uext[q] : 0.0) - V[t] - KN[q];
+    const double u = (Uext ? Uext[q] : 0.0) - V[t] - sum_parts(KN, nz, NT, q);
     Uall[q] = u;
     dU[t][i] = u;
     if (i == 0) dUc[t][0] = u;
@@ -351,8 +407,10 @@ __global__ void batch_potential_kernel(int N, int T, const double* __restrict__ 
 
 // One Broyden update per active tube (workgroup t); step counters on the
 // device so that the launch is identical every iteration.
+template <bool PC>
 __global__ __launch_bounds__(kBroydenThreads) void broyden_batch_kernel(int S, int max_step, double fraction,
                                                                           int norm_type, const double* __restrict__ n_out,
+                                                                          const double* __restrict__ PF,
                                                                           BroydenDev base, size_t state_stride,
                                                                           size_t hist_stride, int* __restrict__ steps,
                                                                           const int* __restrict__ active)
@@ -371,11 +429,12 @@ __global__ __launch_bounds__(kBroydenThreads) void broyden_batch_kernel(int S, i
     b.W = base.W + t * hist_stride;
     b.intermed = base.intermed + t * hist_stride;
     b.out = base.out + (size_t)t * 4;
+    b.PFp = PC ? base.PFp + t * state_stride : nullptr;
     if (m > max_step) {
         if (threadIdx.x == 0) b.out[3] = 1.0;  // step limit reached
         return;
     }
-    broyden_update(S, m, fraction, norm_type, n_out + (size_t)t * S, b);
+    broyden_update<PC>(S, m, fraction, norm_type, n_out + (size_t)t * S, PC ? PF + (size_t)t * S : nullptr, b);
     if (threadIdx.x == 0) {
         steps[t] = m + 2;
         b.out[3] = 0.0;
@@ -393,6 +452,61 @@ __global__ void batch_keep_kernel(int N, int T, const int* __restrict__ keep, co
         Uo[q] = U[q];
         Qo[q] = Q[q];
     }
+}
+
+// R[t][s] = n_in,t[s] - Q[t][s] (the reference's F = n_in - n_out)
+__global__ void batch_residual_kernel(int N, int T, const double* __restrict__ n_in, size_t stride,
+                                      const double* __restrict__ Q, double* __restrict__ R)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)N * T) return;
+    const int t = (int)(q / N), i = (int)(q - (long long)t * N);
+    R[q] = n_in[(size_t)t * stride + i] - Q[q];
+}
+
+// eigen-space scaling of the preconditioner P_t = Z diag(1 / (1 + c_t lam)) Z^T
+__global__ void batch_eigscale_kernel(int N, int T, const double* __restrict__ lam, const double* __restrict__ c,
+                                      const double* __restrict__ parts, int nz, double* __restrict__ X)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long NT = (long long)N * T;
+    if (q >= NT) return;
+    const int t = (int)(q / N), i = (int)(q - (long long)t * N);
+    X[q] = sum_parts(parts, nz, NT, q) / (1.0 + c[t] * lam[i]);
+}
+
+__global__ void batch_sum_parts_kernel(long long NT, const double* __restrict__ parts, int nz, double* __restrict__ X)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < NT) X[q] = sum_parts(parts, nz, NT, q);
+}
+
+// c_t = max(0, mean_s J[t][s]) for the tubes in use: the site-averaged
+// response dQ/dU . 1 (capacitance per site, e per eV); one workgroup per tube
+__global__ __launch_bounds__(256) void batch_rowmean_kernel(int N, const double* __restrict__ J,
+                                                            double* __restrict__ c)
+{
+    __shared__ double red[4];
+    const int t = blockIdx.x;
+    double a = 0.0;
+    for (int s = threadIdx.x; s < N; s += 256) a += J[(size_t)t * N + s];
+    a = wave_sum(a);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = (red[0] + red[1] + red[2] + red[3]) / N;
+        c[t] = m > 0.0 ? m : 0.0;
+    }
+}
+
+// n_in,t -= P R_t (a preconditioned step without history, for the seed)
+__global__ void batch_seed_step_kernel(int N, int T, double* __restrict__ n_in, size_t stride,
+                                       const double* __restrict__ PR)
+{
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)N * T) return;
+    const int t = (int)(q / N), i = (int)(q - (long long)t * N);
+    n_in[(size_t)t * stride + i] -= PR[q];
 }
 
 class BatchScf {
@@ -421,14 +535,15 @@ class BatchScf {
         }
         coulomb_kernel(tubes[0]->t.ptd, p, K);
         M = p.Broyden_threshold_maxstep;
-        state_stride = 6 * (size_t)N;
+        state_stride = 7 * (size_t)N;
         hist_stride = 2 * (size_t)(M + 1) * N + (M + 1);
         int rc;
         NEGF_HIP_TRY(hipStreamCreate(&st));
         if ((rc = d_K.ensure(sizeof(double) * K.size()))) return rc;
         if ((rc = d_Uext.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_V.ensure(sizeof(double) * T))) return rc;
-        if ((rc = d_KN.ensure(sizeof(double) * N * T))) return rc;
+        nz = gemm_splits(N, T);
+        if ((rc = d_KN.ensure(sizeof(double) * N * T * nz))) return rc;
         if ((rc = d_U.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_Q.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_Uo.ensure(sizeof(double) * N * T))) return rc;
@@ -468,23 +583,18 @@ class BatchScf {
         b.W = h + (size_t)(M + 1) * N;
         b.intermed = h + 2 * (size_t)(M + 1) * N;
         b.out = d_out.as<double>();
+        b.PFp = s0 + 6 * (size_t)N;
         return b;
     }
     int enqueue_iteration()
     {
-        dim3 g((N + 31) / 32, (T + 31) / 32);
-        hipLaunchKernelGGL(dgemm_kx_kernel, g, dim3(256), 0, st, N, T, d_K.as<double>(), base().n_in, state_stride,
-                           d_KN.as<double>());
-        const long long NT = (long long)N * T;
-        hipLaunchKernelGGL(batch_potential_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
-                           has_ext ? d_Uext.as<double>() : nullptr, d_V.as<double>(), d_KN.as<double>(),
-                           d_ptrU.as<double*>(), d_ptrUc.as<double*>(), d_U.as<double>());
-        NEGF_HIP_TRY(hipGetLastError());
-        int rc = batch_enqueue(batch, d_Q.as<double>(), st);
+        int rc = enqueue_potential();
         if (rc) return rc;
-        hipLaunchKernelGGL(broyden_batch_kernel, dim3(T), dim3(kBroydenThreads), 0, st, N, M, p.Broyden_fraction,
-                           p.Broyden_norm_type, d_Q.as<double>(), base(), state_stride, hist_stride,
-                           d_steps.as<int>(), d_active.as<int>());
+        rc = batch_enqueue(batch, d_Q.as<double>(), st, d_active.as<int>());
+        if (rc) return rc;
+        hipLaunchKernelGGL(broyden_batch_kernel<false>, dim3(T), dim3(kBroydenThreads), 0, st, N, M,
+                           p.Broyden_fraction, p.Broyden_norm_type, d_Q.as<double>(), nullptr, base(), state_stride,
+                           hist_stride, d_steps.as<int>(), d_active.as<int>());
         NEGF_HIP_TRY(hipGetLastError());
         NEGF_HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, sizeof(double) * 4 * T, hipMemcpyDeviceToHost, st));
         return NEGF_OK;
@@ -493,6 +603,8 @@ class BatchScf {
     {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         gexec = nullptr;
+        if (gexec_pc) (void)hipGraphExecDestroy(gexec_pc);
+        gexec_pc = nullptr;
     }
     // Run every tube to Broyden_Norm <= Broyden_max_norm (or its step
     // limit); per-tube iteration counts and final norms.
@@ -586,8 +698,210 @@ class BatchScf {
         return NEGF_OK;
     }
 
+    // ---- surrogate-seeded, preconditioned iteration
+    int set_preconditioner(const double* Z, const double* lam)
+    {
+        int rc;
+        const size_t NN = (size_t)N * N;
+        if ((rc = d_Z.ensure(sizeof(double) * NN))) return rc;
+        if ((rc = d_Zt.ensure(sizeof(double) * NN))) return rc;
+        if ((rc = d_lam.ensure(sizeof(double) * N))) return rc;
+        if ((rc = d_c.ensure(sizeof(double) * T))) return rc;
+        if ((rc = d_R.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_Rh.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_PF.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_ones.ensure(sizeof(double) * N * T))) return rc;
+        if ((rc = d_J.ensure(sizeof(double) * N * T))) return rc;
+        std::vector<double> zt(NN);
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) zt[(size_t)i * N + j] = Z[(size_t)j * N + i];
+        NEGF_HIP_TRY(hipMemcpy(d_Z.p, Z, sizeof(double) * NN, hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_Zt.p, zt.data(), sizeof(double) * NN, hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_lam.p, lam, sizeof(double) * N, hipMemcpyHostToDevice));
+        if ((rc = fill_device(d_ones.as<double>(), (long long)N * T, 1.0, st))) return rc;
+        if ((rc = fill_device(d_c.as<double>(), T, 0.0, st))) return rc;
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        has_pc = true;
+        drop_graph();
+        return NEGF_OK;
+    }
+    // U_t = U_ext,t - V_t - K n_in,t into the tubes and d_U
+    int enqueue_potential()
+    {
+        launch_dgemm(N, T, nz, d_K.as<double>(), base().n_in, state_stride, d_KN.as<double>(), st);
+        const long long NT = (long long)N * T;
+        hipLaunchKernelGGL(batch_potential_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
+                           has_ext ? d_Uext.as<double>() : nullptr, d_V.as<double>(), d_KN.as<double>(), nz,
+                           d_ptrU.as<double*>(), d_ptrUc.as<double*>(), d_U.as<double>());
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
+    // d_PF = P (n_in - Q) from the charge Q (T x N)
+    int enqueue_precondition(const double* Q)
+    {
+        const long long NT = (long long)N * T;
+        const dim3 g((unsigned)((NT + 255) / 256));
+        hipLaunchKernelGGL(batch_residual_kernel, g, dim3(256), 0, st, N, T, base().n_in, state_stride, Q,
+                           d_R.as<double>());
+        launch_dgemm(N, T, nz, d_Zt.as<double>(), d_R.as<double>(), (size_t)N, d_KN.as<double>(), st);
+        hipLaunchKernelGGL(batch_eigscale_kernel, g, dim3(256), 0, st, N, T, d_lam.as<double>(), d_c.as<double>(),
+                           d_KN.as<double>(), nz, d_Rh.as<double>());
+        launch_dgemm(N, T, nz, d_Z.as<double>(), d_Rh.as<double>(), (size_t)N, d_KN.as<double>(), st);
+        hipLaunchKernelGGL(batch_sum_parts_kernel, g, dim3(256), 0, st, NT, d_KN.as<double>(), nz,
+                           d_PF.as<double>());
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
+    // c_t from the surrogate's tangent dQ/dU . 1 at the present d_U
+    int enqueue_capacitance()
+    {
+        int rc = sg->forward(T, d_U.as<double>(), has_feat ? d_feat.as<double>() : nullptr, st);
+        if (!rc) rc = sg->jvp(T, d_ones.as<double>(), d_J.as<double>(), st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(batch_rowmean_kernel, dim3(T), dim3(256), 0, st, N, d_J.as<double>(), d_c.as<double>());
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
+    int enqueue_iteration_pc()
+    {
+        int rc = enqueue_potential();
+        if (!rc) rc = batch_enqueue(batch, d_Q.as<double>(), st, d_active.as<int>());
+        if (!rc) rc = enqueue_precondition(d_Q.as<double>());
+        if (rc) return rc;
+        hipLaunchKernelGGL(broyden_batch_kernel<true>, dim3(T), dim3(kBroydenThreads), 0, st, N, M,
+                           p.Broyden_fraction, p.Broyden_norm_type, d_Q.as<double>(), d_PF.as<double>(), base(),
+                           state_stride, hist_stride, d_steps.as<int>(), d_active.as<int>());
+        NEGF_HIP_TRY(hipGetLastError());
+        NEGF_HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, sizeof(double) * 4 * T, hipMemcpyDeviceToHost, st));
+        return NEGF_OK;
+    }
+    // Seeded, preconditioned run: seed_iters preconditioned steps on the
+    // surrogate charge (a fixed-point iteration of the surrogate SCF), then
+    // the Broyden loop on the exact NEGF charge with the mixing term P F.  The
+    // capacitances come from the surrogate at the start, half way through
+    // the seed, at the seed, and (with a fresh history) at iteration
+    // refresh_iter of the exact loop if refresh_iter > 0.
+    int run_pc(const double* V, int seed_iters, int refresh_iter, double* U_out, double* Q_out, int* iters,
+               double* norms, int* total_iters, double* cap)
+    {
+        int rc;
+        NEGF_HIP_TRY(hipMemcpyAsync(d_V.p, V, sizeof(double) * T, hipMemcpyHostToDevice, st));
+        std::vector<int> steps(T, 1), active(T, 1), keep(T, 0), done_it(T, -1);
+        std::vector<double> fin(T, 0.0);
+        NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        if ((rc = enqueue_potential())) return rc;
+        if ((rc = enqueue_capacitance())) return rc;
+        const long long NT = (long long)N * T;
+        for (int k = 0; k < seed_iters; ++k) {
+            // the capacitances follow the seed once, half way
+            if (k > 0 && k == seed_iters / 2 && (rc = enqueue_capacitance())) return rc;
+            if ((rc = sg->forward(T, d_U.as<double>(), has_feat ? d_feat.as<double>() : nullptr, st))) return rc;
+            if ((rc = sg->output(T, d_Q.as<double>(), st))) return rc;
+            if ((rc = enqueue_precondition(d_Q.as<double>()))) return rc;
+            hipLaunchKernelGGL(batch_seed_step_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
+                               base().n_in, state_stride, d_PF.as<double>());
+            NEGF_HIP_TRY(hipGetLastError());
+            if ((rc = enqueue_potential())) return rc;
+        }
+        if (seed_iters > 0 && (rc = enqueue_capacitance())) return rc;
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        int it = 0, remaining = T;
+        bool overflow = false;
+        while (remaining > 0) {
+            if (it > M + 1) {
+                overflow = true;
+                break;
+            }
+            if (it > 0 && it == refresh_iter) {
+                // new capacitances at the present potential; fresh history, n_in kept
+                if ((rc = enqueue_capacitance())) return rc;
+                for (int t = 0; t < T; ++t) steps[t] = 1;
+                NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+            }
+            bool rebuilt = false;
+            if ((rc = batch_prepare(batch, it, &rebuilt))) return rc;
+            if (use_graph) {
+                if (!gexec_pc || rebuilt) {
+                    if (gexec_pc) (void)hipGraphExecDestroy(gexec_pc);
+                    gexec_pc = nullptr;
+                    NEGF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    const int erc = enqueue_iteration_pc();
+                    hipGraph_t g = nullptr;
+                    const hipError_t e = hipStreamEndCapture(st, &g);
+                    if (erc) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return erc;
+                    }
+                    if (e != hipSuccess) {
+                        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+                        return NEGF_EDEVICE;
+                    }
+                    const hipError_t e2 = hipGraphInstantiate(&gexec_pc, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    if (e2 != hipSuccess) {
+                        gexec_pc = nullptr;
+                        set_error("hipGraphInstantiate: %s", hipGetErrorString(e2));
+                        return NEGF_EDEVICE;
+                    }
+                }
+                NEGF_HIP_TRY(hipGraphLaunch(gexec_pc, st));
+            } else if ((rc = enqueue_iteration_pc())) {
+                return rc;
+            }
+            NEGF_HIP_TRY(hipStreamSynchronize(st));
+            const double* o = h_out.as<double>();
+            bool any = false;
+            for (int t = 0; t < T; ++t) {
+                keep[t] = 0;
+                if (!active[t]) continue;
+                if (o[4 * t + 3] != 0.0) {
+                    overflow = true;
+                    active[t] = 0;
+                    --remaining;
+                    continue;
+                }
+                if (o[4 * t] <= p.Broyden_max_norm) {
+                    keep[t] = 1;
+                    active[t] = 0;
+                    done_it[t] = it + 1;
+                    fin[t] = o[4 * t];
+                    --remaining;
+                    any = true;
+                }
+            }
+            if (any) {
+                NEGF_HIP_TRY(hipMemcpyAsync(d_keep.p, keep.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(batch_keep_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
+                                   d_keep.as<int>(), d_U.as<double>(), d_Q.as<double>(), d_Uo.as<double>(),
+                                   d_Qo.as<double>());
+                NEGF_HIP_TRY(hipGetLastError());
+                NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+            }
+            ++it;
+        }
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        if (U_out) NEGF_HIP_TRY(hipMemcpy(U_out, d_Uo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        if (Q_out) NEGF_HIP_TRY(hipMemcpy(Q_out, d_Qo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        if (cap) NEGF_HIP_TRY(hipMemcpy(cap, d_c.p, sizeof(double) * T, hipMemcpyDeviceToHost));
+        if (iters) std::copy(done_it.begin(), done_it.end(), iters);
+        if (norms) std::copy(fin.begin(), fin.end(), norms);
+        if (total_iters) *total_iters = it;
+        if (overflow) {
+            set_error("Broyden_Step has exceeded the Broyden_Threshold_MaxStep for some tube");
+            return NEGF_ENONCONV;
+        }
+        return NEGF_OK;
+    }
+
+    Surrogate* sg = nullptr;
+    bool has_pc = false, has_feat = false;
+    DevBuf d_Z, d_Zt, d_lam, d_c, d_R, d_Rh, d_PF, d_ones, d_J, d_feat;
+    hipGraphExec_t gexec_pc = nullptr;
+
     negf_batch* batch = nullptr;
     int T = 0, N = 0, M = 0;
+    int nz = 1;  // K-split of the GEMMs (partials in d_KN)
     negf_scf_params p{};
     std::vector<double> K;
     std::vector<double*> hU, hUc;
@@ -920,6 +1234,72 @@ int negf_batch_scf_use_graph(negf_batch_scf* s, int on)
     return NEGF_OK;
 }
 
+int negf_batch_scf_get_kernel(const negf_batch_scf* s, double* K)
+{
+    clear_error();
+    if (!s || !K) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    std::copy(s->s.K.begin(), s->s.K.end(), K);
+    return NEGF_OK;
+}
+
+int negf_batch_scf_set_preconditioner(negf_batch_scf* s, const double* Z, const double* lam)
+{
+    clear_error();
+    if (!s || !Z || !lam) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    return s->s.set_preconditioner(Z, lam);
+}
+
+int negf_batch_scf_set_surrogate(negf_batch_scf* s, negf_surrogate* sg, const double* feat)
+{
+    clear_error();
+    if (!s || !sg) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    BatchScf& b = s->s;
+    if (sg->s.N != b.N) {
+        set_error("surrogate has %d sites, the tubes %d", sg->s.N, b.N);
+        return NEGF_EARG;
+    }
+    b.sg = &sg->s;
+    b.has_feat = feat != nullptr;
+    if (feat) {
+        int rc = b.d_feat.ensure(sizeof(double) * b.T);
+        if (rc) return rc;
+        NEGF_HIP_TRY(hipMemcpy(b.d_feat.p, feat, sizeof(double) * b.T, hipMemcpyHostToDevice));
+    }
+    b.drop_graph();
+    return NEGF_OK;
+}
+
+int negf_batch_scf_run_surrogate(negf_batch_scf* s, const double* V_gate, int seed_iters, int refresh_iter,
+                                 double* U_out, double* Q_out, int* iters, double* final_norms,
+                                 int* total_iterations, double* capacitance)
+{
+    clear_error();
+    if (!s || !V_gate || seed_iters < 0) {
+        set_error("invalid arguments");
+        return NEGF_EARG;
+    }
+    BatchScf& b = s->s;
+    if (!b.sg || !b.has_pc) {
+        set_error("set a surrogate and a preconditioner basis first");
+        return NEGF_EARG;
+    }
+    int rc = fill_device(b.d_hist.as<double>(), (long long)(b.hist_stride * b.T), 0.0, b.st);
+    if (rc) return rc;
+    for (int t = 0; t < b.T; ++t)
+        if ((rc = fill_device(b.d_state.as<double>() + t * b.state_stride + b.N, 6LL * b.N, 0.0, b.st))) return rc;
+    return b.run_pc(V_gate, seed_iters, refresh_iter, U_out, Q_out, iters, final_norms, total_iterations,
+                    capacitance);
+}
+
 int negf_batch_scf_run(negf_batch_scf* s, const double* V_gate, double* U_out, double* Q_out, int* iters,
                        double* final_norms, int* total_iterations)
 {
@@ -933,7 +1313,7 @@ int negf_batch_scf_run(negf_batch_scf* s, const double* V_gate, double* U_out, d
     int rc = fill_device(b.d_hist.as<double>(), (long long)(b.hist_stride * b.T), 0.0, b.st);
     if (rc) return rc;
     for (int t = 0; t < b.T; ++t)
-        if ((rc = fill_device(b.d_state.as<double>() + t * b.state_stride + b.N, 5LL * b.N, 0.0, b.st))) return rc;
+        if ((rc = fill_device(b.d_state.as<double>() + t * b.state_stride + b.N, 6LL * b.N, 0.0, b.st))) return rc;
     return b.run(V_gate, U_out, Q_out, iters, final_norms, total_iterations);
 }
 

@@ -54,7 +54,7 @@ struct WTube {
     int group0;           // first global chain group of this tube
     long long part_off;   // this tube's partial rows (groups x N)
     int sample[3];        // non-equilibrium integrand sample sites (-1: none)
-    int pad;
+    int ext;              // index of the tube in its caller (the active mask)
     const double* Ucp;    // device contact potentials (2 values), or null: Uc0/Uc1
 };
 
@@ -76,7 +76,13 @@ struct WLaunch {
     int kind;                 // ChainKind of the launch
     const int* item_list;     // sweep launches: items of this launch (or null: all)
     int nlist;
+    const int* tube_active;   // optional [ext]: 0 skips the tube's work (its outputs stay as they were)
 };
+
+__device__ __forceinline__ bool tube_skipped(const WLaunch& lp, const WTube& tube)
+{
+    return lp.tube_active && !lp.tube_active[__builtin_amdgcn_readfirstlane(tube.ext)];
+}
 
 __device__ __forceinline__ int exp_of(double m) { return m == 0.0 ? 0 : __builtin_amdgcn_frexp_exp(m); }
 __device__ __forceinline__ double amax4(double2 a, double2 b)
@@ -234,6 +240,7 @@ __global__ __launch_bounds__(256, 8) void transfer_kernel(WLaunch lp)
     if (item >= lp.nitems) return;
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
+    if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
     const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int lo = min(tube.N, ic.s * tube.L);
@@ -352,6 +359,7 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     const int t = lp.sblk_tube[gi];
     const int g = lp.sblk_chain[gi];
     const WTube tube = lp.tubes[t];
+    if (tube_skipped(lp, tube)) return;  // uniform over the block
     const int P = tube.P, L = tube.L, N = tube.N;
     const int lc = g * 64 + lane;
     const bool valid = lc < tube.nchain;
@@ -593,6 +601,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
     double* red = red_all[wv];
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
+    if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
     const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int N = tube.N;
@@ -740,6 +749,7 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     double* red = red_all[wv];
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
+    if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
     const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int N = tube.N;
@@ -837,6 +847,7 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_neq_full_kernel(WL
     double* red = red_all[wv];
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
+    if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
     const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int N = tube.N;
@@ -970,10 +981,13 @@ __global__ __launch_bounds__(256) void reduce_rows1_kernel(const double* __restr
                                                            const long long* __restrict__ src_off,
                                                            const int* __restrict__ nrows,
                                                            const int* __restrict__ Ns, double* __restrict__ dst,
-                                                           const long long* __restrict__ dst_off)
+                                                           const long long* __restrict__ dst_off,
+                                                           const WTube* __restrict__ tubes,
+                                                           const int* __restrict__ tube_active)
 {
     __shared__ double red[16][17];
     const int t = blockIdx.y;
+    if (tube_active && !tube_active[tubes[t].ext]) return;  // uniform over the block
     const int N = Ns[t];
     const int sl = threadIdx.x & 15;   // site within the block
     const int k = threadIdx.x >> 4;    // row slice

@@ -268,6 +268,36 @@ class BatchScf:
         return U.reshape(self.T, self.N), Q.reshape(self.T, self.N), it, nrm, tot.value
 
 
+    # ---- surrogate-seeded, preconditioned runs
+    def kernel(self):
+        K = np.zeros(self.N * self.N)
+        _lib.check(self.lib.negf_batch_scf_get_kernel(self.h, _dp(K)))
+        return K.reshape(self.N, self.N)
+
+    def set_surrogate(self, surrogate, feat=None):
+        """Attach a charge surrogate (electrostatic_amd.surrogate.Surrogate of
+        the same N) and the eigenbasis of the shared kernel (host eigen
+        decomposition, once per geometry)."""
+        lam, Z = np.linalg.eigh(self.kernel())
+        Z = np.ascontiguousarray(Z)
+        _lib.check(self.lib.negf_batch_scf_set_preconditioner(self.h, _dp(Z), _dp(np.ascontiguousarray(lam))))
+        f = None if feat is None else np.ascontiguousarray(feat, np.float64)
+        _lib.check(self.lib.negf_batch_scf_set_surrogate(self.h, surrogate.h, None if f is None else _dp(f)))
+        self._surrogate = surrogate
+
+    def run_surrogate(self, V_gate, seed_iters=6, refresh_iter=0):
+        V = np.ascontiguousarray(V_gate, np.float64)
+        U = np.zeros(self.T * self.N)
+        Q = np.zeros(self.T * self.N)
+        it = np.zeros(self.T, np.int32)
+        nrm = np.zeros(self.T)
+        cap = np.zeros(self.T)
+        tot = C.c_int()
+        _lib.check(self.lib.negf_batch_scf_run_surrogate(self.h, _dp(V), seed_iters, refresh_iter, _dp(U), _dp(Q),
+                                                         _ip(it), _dp(nrm), C.byref(tot), _dp(cap)))
+        return U.reshape(self.T, self.N), Q.reshape(self.T, self.N), it, nrm, tot.value, cap
+
+
 class Broyden:
     """Broyden modified-second mixer on the GPU (Broyden_Second_Parallel.cpp)."""
 

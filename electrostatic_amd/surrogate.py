@@ -126,3 +126,44 @@ class Surrogate:
         _lib.check(self.lib.negf_surrogate_train(self.h, S, _ptr(dU), _ptr(df), _ptr(dQ), epochs, batch, lr,
                                                  lr_end, seed, C.byref(loss)))
         return loss.value
+
+
+def sample_potentials(n, N, seed, gate=(-0.2, 1.2), bump_amp=0.5, field_amp=0.3):
+    """Seeded smooth potentials (eV) [n, N] of the kind an SCF visits: a
+    uniform shift -V (V uniform in gate), four Gaussian bumps (amplitudes up
+    to bump_amp, widths N/20..N/5) and a random smooth field of 8 Fourier
+    modes (amplitude field_amp)."""
+    rng = np.random.default_rng(seed)
+    x = np.arange(N)
+    U = np.empty((n, N))
+    for k in range(n):
+        u = -rng.uniform(*gate) * np.ones(N)
+        for _ in range(4):
+            c, w, a = rng.uniform(0, N), rng.uniform(N / 20, N / 5), rng.uniform(-bump_amp, bump_amp)
+            u += a * np.exp(-0.5 * ((x - c) / w) ** 2)
+        f = np.zeros(N)
+        for m in range(1, 9):
+            f += rng.standard_normal() / m * np.sin(np.pi * m * x / N + rng.uniform(0, 2 * np.pi))
+        u += field_amp * f / max(np.abs(f).max(), 1e-12)
+        U[k] = u
+    return U
+
+
+def negf_pairs(batch, U, device="cuda"):
+    """Exact NEGF charges for the potentials U [S, N], computed T at a time
+    with the T equal tubes of a negf.Batch (the contact potentials follow
+    U[0], U[N-1]); returns (U, Q) as device tensors [S, N]."""
+    tubes = batch.tubes
+    T, N = len(tubes), tubes[0].N
+    S = U.shape[0]
+    dU = torch.as_tensor(np.ascontiguousarray(U, np.float64), device=device)
+    dQ = torch.empty_like(dU)
+    buf = torch.empty(T * N, dtype=torch.float64, device=device)
+    for s0 in range(0, S, T):
+        n = min(T, S - s0)
+        for t in range(T):
+            tubes[t].set_potential(U[s0 + min(t, n - 1)])
+        batch.solve_device(1, buf.data_ptr())
+        dQ[s0:s0 + n] = buf.view(T, N)[:n]
+    torch.cuda.synchronize()
+    return dU, dQ

@@ -399,6 +399,35 @@ int negf_surrogate_train(negf_surrogate *s, int nsamples, const double *d_U,
                          unsigned long long seed, double *last_loss);
 
 /* ------------------------------------------------------------------------
+ * Surrogate-seeded, preconditioned batched SCF (no reference counterpart).
+ * The Broyden mixing term fraction * F of the reference update
+ * (Broyden_Second_Parallel.cpp:172-186, :106-135) becomes P_t F with the
+ * Kerker-type preconditioner P_t = Z diag(1 / (1 + c_t lam)) Z^T, where
+ * K = Z diag(lam) Z^T is the shared electrostatic kernel and c_t the
+ * site-averaged capacitance dQ/dU . 1 that the charge surrogate predicts
+ * for tube t.  Every iterate is still judged by the exact NEGF residual
+ * and the same Broyden_max_norm.
+ * ------------------------------------------------------------------------ */
+
+/* The shared kernel K (N x N, row-major) of a batched SCF. */
+int negf_batch_scf_get_kernel(const negf_batch_scf *s, double *K);
+/* Eigenbasis of K: Z (N x N row-major, column k = eigenvector k), lam[N]. */
+int negf_batch_scf_set_preconditioner(negf_batch_scf *s, const double *Z, const double *lam);
+/* The surrogate (same N) and its per-tube feature feat[T] (host, or NULL). */
+int negf_batch_scf_set_surrogate(negf_batch_scf *s, negf_surrogate *sg, const double *feat);
+/* Run all tubes to Broyden_Norm <= Broyden_max_norm from the present input
+ * charge: seed_iters preconditioned fixed-point steps on the surrogate
+ * charge first (the seed), then Broyden with P_t F on the exact NEGF charge.
+ * c_t comes from the surrogate at the start, half way through the seed, at
+ * the seed, and at iteration refresh_iter of the exact loop (fresh history,
+ * n_in kept; <= 0: never).  Outputs as negf_batch_scf_run; capacitance [T]
+ * (may be NULL): the c_t in use at the end. */
+int negf_batch_scf_run_surrogate(negf_batch_scf *s, const double *V_gate, int seed_iters,
+                                 int refresh_iter, double *U_out, double *Q_out, int *iters,
+                                 double *final_norms, int *total_iterations,
+                                 double *capacitance);
+
+/* ------------------------------------------------------------------------
  * Output writers (NEGF_Common.cpp:1343-1418, 3044-3112, 4223-4254):
  * <prefix>_Qout.dat, <prefix>_U.dat, <prefix>_norm.dat in the reference's
  * column formats.

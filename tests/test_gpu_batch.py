@@ -150,3 +150,41 @@ def test_batched_scf_matches_oracle_per_tube(dev):
     assert np.abs(U2 - U).max() < 1e-10
     scf.close()
     batch.close()
+
+
+def test_surrogate_preconditioned_scf_matches_plain(dev):
+    # a small surrogate trained on NEGF pairs of these tubes; the seeded,
+    # preconditioned run must reach the same fixed point as the plain run
+    from electrostatic_amd.surrogate import Surrogate, negf_pairs, sample_potentials
+    T, cells = 4, 40
+    tubes = [negf.Tube(num_unitcells=cells, num_modes=1, contact_mu=(-1.0, -1.0),
+                       eq_integration_pts=(30, 30, 64), noneq_integration_pts=(200,),
+                       E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0) for _ in range(T)]
+    N = tubes[0].N
+    batch = negf.Batch(tubes)
+    U_tr = sample_potentials(256, N, seed=1)
+    dU, dQ = negf_pairs(batch, U_tr, device=dev)
+    sg = Surrogate(N, seed=3)
+    loss = sg.train(dU, dQ, epochs=30, batch=16, lr=3e-3, lr_end=3e-4, seed=2)
+    assert np.isfinite(loss)
+    sp = dict(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+    scf = negf.BatchScf(batch, Broyden_fraction=0.1, Broyden_max_norm=1e-8,
+                        Broyden_threshold_maxstep=200, **sp)
+    U_ext = np.stack([gaussian_bumps(N, seed=40 + t, amp=0.3) for t in range(T)])
+    V = np.array([0.0, 0.3, 0.6, 1.0])
+    scf.set_external_potential(U_ext)
+    scf.set_input_charge(np.zeros((T, N)))
+    U0, Q0, it0, nrm0, tot0 = scf.run(V)
+    assert np.all(nrm0 <= 1e-8)
+    scf.set_surrogate(sg)
+    for seed_iters, graph in ((0, True), (1, True), (1, False)):
+        scf.use_graph(graph)
+        scf.set_input_charge(np.zeros((T, N)))
+        U1, Q1, it1, nrm1, tot1, cap = scf.run_surrogate(V, seed_iters=seed_iters, refresh_iter=3)
+        assert np.all(nrm1 <= 1e-8)
+        assert np.all(cap > 0)
+        assert np.abs(U1 - U0).max() < 1e-6, (seed_iters, np.abs(U1 - U0).max())
+        assert np.abs(Q1 - Q0).max() < 1e-6
+    scf.close()
+    sg.close()
+    batch.close()
