@@ -185,6 +185,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     long long poff = 0, toff = 0;
     int items = 0;
     max_N_ = 0;
+    max_P_ = 0;
     max_groups_ = 0;
     for (int t = 0; t < ntubes_; ++t) {
         const RgfTube& tb = tubes[t];
@@ -246,6 +247,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         toff += (long long)nch * tb.N;
         rdst.push_back(tb.out_off);
         max_N_ = std::max(max_N_, tb.N);
+        max_P_ = std::max(max_P_, P);
         max_groups_ = std::max(max_groups_, G);
     }
     nitems_ = items;
@@ -338,9 +340,11 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         }
         NEGF_HIP_TRY(hipEventRecord(ev_start_[ev_used_], st));
     }
-    const int nblk = (nitems_ + 3) / 4;
-    hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3(nblk), dim3(256), 0, st, lp);
-    hipLaunchKernelGGL(scan_kernel, dim3(nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
+    hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((nitems_ + 3) / 4), dim3(256), 0, st, lp);
+    if (max_P_ <= 4 * SCAN_WAVES)
+        hipLaunchKernelGGL(scan_kernel<4>, dim3(nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
+    else
+        hipLaunchKernelGGL(scan_kernel<0>, dim3(nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
     for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
         if (fast_n_[ns] == 0) continue;
         WLaunch lf = lp;

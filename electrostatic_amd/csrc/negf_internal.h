@@ -97,7 +97,7 @@ class RgfPlan {
     int kind_ = 0;
     bool write_G_ = false;
     int nitems_ = 0, nchains_ = 0, ntubes_ = 0, nsblk_ = 0;
-    int max_N_ = 0, max_groups_ = 0, max_chunks_ = 0;
+    int max_N_ = 0, max_P_ = 0, max_groups_ = 0, max_chunks_ = 0;
     int reduce_chunk_ = 16;
     ChainDesc* d_chains_ = nullptr;
     DecParams dec_;

@@ -298,7 +298,7 @@ __global__ __launch_bounds__(256, 8) void transfer_kernel(WLaunch lp)
 // (phase B), then it runs both states through its own range writing the
 // per-segment states (phase C).  The sequential depth is about 3 P /
 // SCAN_WAVES matrix steps instead of 2 P.
-#define SCAN_WAVES 16
+#define SCAN_WAVES 8
 
 struct Mat2 {
     double2 a, b, c, d;  // [[a, b], [c, d]]
@@ -349,6 +349,10 @@ __device__ __forceinline__ void mat2_apply_inv(const Mat2& m, double idet, doubl
     renorm(x, y, ex);
 }
 
+// CACHE > 0: every wave's range has at most CACHE segments (P <= CACHE *
+// SCAN_WAVES for every tube of the launch), and the transfer matrices stay in
+// registers from phase A to phase C instead of being read again.
+template <int CACHE>
 __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int ngroups)
 {
     __shared__ double xm[SCAN_WAVES][9][64];
@@ -390,18 +394,26 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     M.c = make_double2(0.0, 0.0);
     M.d = make_double2(1.0, 0.0);
     M.e = 0;
-    // (registers cannot hold the range for phase C at 1024 threads per block:
-    // phase C reloads the transfer matrices)
-    const bool cached = false;
-    Mat2 Tk[4];
-    for (int c = s0; c < s1; c += 4) {
-        // issue the loads of up to four segments before the products
+    constexpr bool cached = CACHE > 0;
+    constexpr int NT = cached ? CACHE : 4;
+    Mat2 Tk[NT];
+    if (cached) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (c + q < s1) Tk[q] = load_T(lp, item0 + c + q, lane);
+        for (int q = 0; q < NT; ++q)
+            if (s0 + q < s1) Tk[q] = load_T(lp, item0 + s0 + q, lane);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (c + q < s1) M = mat2_mul(Tk[q], M);
+        for (int q = 0; q < NT; ++q)
+            if (s0 + q < s1) M = mat2_mul(Tk[q], M);
+    } else {
+        for (int c = s0; c < s1; c += 4) {
+            // issue the loads of up to four segments before the products
+#pragma unroll
+            for (int q = 0; q < NT; ++q)
+                if (c + q < s1) Tk[q] = load_T(lp, item0 + c + q, lane);
+#pragma unroll
+            for (int q = 0; q < NT; ++q)
+                if (c + q < s1) M = mat2_mul(Tk[q], M);
+        }
     }
     xm[w][0][lane] = M.a.x;
     xm[w][1][lane] = M.a.y;
@@ -437,22 +449,29 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     }
 
     // phase C: per-segment states through this wave's range
-    for (int c = s0; c < s1; c += 4) {
-        Mat2 Tc[4];
+    auto u_step = [&](int s, const Mat2& Ts) {
+        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
+        S[0 * 64] = ux.x;
+        S[1 * 64] = ux.y;
+        S[2 * 64] = uy.x;
+        S[3 * 64] = uy.y;
+        S[4 * 64] = (double)ue;
+        mat2_apply(Ts, ux, uy, ue);
+    };
+    if (cached) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (c + q < s1) Tc[q] = cached ? Tk[q] : load_T(lp, item0 + c + q, lane);
+        for (int q = 0; q < NT; ++q)
+            if (s0 + q < s1) u_step(s0 + q, Tk[q]);
+    } else {
+        for (int c = s0; c < s1; c += 4) {
+            Mat2 Tc[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (c + q < s1) {
-                double* S = lp.St + (item0 + c + q) * 10 * 64 + lane;
-                S[0 * 64] = ux.x;
-                S[1 * 64] = ux.y;
-                S[2 * 64] = uy.x;
-                S[3 * 64] = uy.y;
-                S[4 * 64] = (double)ue;
-                mat2_apply(Tc[q], ux, uy, ue);
-            }
+            for (int q = 0; q < 4; ++q)
+                if (c + q < s1) Tc[q] = load_T(lp, item0 + c + q, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c + q < s1) u_step(c + q, Tc[q]);
+        }
     }
     if (w == SCAN_WAVES - 1 && valid) {
         // this wave's u now sits at the end of the chain: (u_N, u_{N-1})
@@ -483,7 +502,7 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     };
     if (cached) {
 #pragma unroll
-        for (int q = 3; q >= 0; --q)
+        for (int q = NT - 1; q >= 0; --q)
             if (s0 + q < s1) v_step(s0 + q, Tk[q]);
     } else {
         for (int c = s1 - 1; c >= s0; c -= 4) {
