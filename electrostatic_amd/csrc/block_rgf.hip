@@ -115,97 +115,126 @@ __device__ __forceinline__ unsigned long long pivot_key(double m, int i)
     return ((unsigned long long)__double_as_longlong(m) & ~31ull) | (unsigned long long)(31 - i);
 }
 
-// Inverse of the complex matrix (Mre, Mim), in place, by Gauss-Jordan with
-// partial pivoting (row swaps, undone as column swaps at the end); (Wre, Wim)
-// is a second 32 x 32 buffer that the steps alternate with, so that one
-// barrier per step suffices.  Thread t owns column j = t % 32 of rows
-// t / 32 + 8 r (r = 0..3): each step reads the old matrix (the pivot row swap
-// folded into the reads) and writes its four new entries to the other
-// buffer.  The threads of the next pivot column leave one candidate key each
-// in keys[2][8], so the pivot search is a max over 8 integers.
-__device__ void gj_inverse(double* Mre, double* Mim, double* Wre, double* Wim, unsigned long long* keys, int* perm)
+// 64-bit value of another lane by a DPP move (CTRL: a compile-time DPP
+// control word; all lanes active).
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp64(unsigned long long v)
 {
-    const int tid = threadIdx.x, j = tid & 31, g = tid >> 5;
-    int roff[4];
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v & 0xffffffffull), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xF, 0xF, false);
+    return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+
+// max over each aligned group of 8 lanes: quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], then row_half_mirror (lane i <- lane 7 - i within 8 lanes)
+__device__ __forceinline__ unsigned long long max8(unsigned long long v)
+{
+    unsigned long long o = dpp64<0xB1>(v);
+    v = o > v ? o : v;
+    o = dpp64<0x4E>(v);
+    v = o > v ? o : v;
+    o = dpp64<0x141>(v);
+    return o > v ? o : v;
+}
+
+// Inverse of the complex matrix (Mre, Mim), in place, by Gauss-Jordan with
+// partial pivoting.  The matrix lives in registers during the elimination:
+// thread t owns column j = t / 8 and rows t % 8 + 8 r (r = 0..3), so the 8
+// owners of a column are 8 lanes of one wavefront.  Rows are never swapped:
+// step k takes as pivot the largest |a_ik| among the rows not yet used (ties
+// to the smallest row), and the result is read back through the pivot order,
+// A^-1[i][j] = M[perm[i]][q(j)] with perm[q(j)] = j.  Per step the 8 owners
+// of column k publish it (and the pivot row) through LDS behind one barrier;
+// every thread takes the pivot row's entry of its column from the owning
+// lane of its own wavefront by a shuffle.  The owner of the pivot entry
+// publishes 1 / pivot with the pivot row, so no thread divides after the
+// barrier.  (Wre, Wim) provide the column buffers (2 x 32 entries each) and
+// the reciprocal pivots (Wre[64..67]).
+__device__ void gj_inverse(double* Mre, double* Mim, double* Wre, double* Wim, int* perm)
+{
+    const int t = threadIdx.x, j = t >> 3, c = t & 7, lane = t & 63;
+    double ar[4], ai[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) roff[r] = (g + 8 * r) * BLD;
-    if (j == 0) {
-        unsigned long long kb = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const unsigned long long kk = pivot_key(mag2(Mre[roff[r]], Mim[roff[r]]), g + 8 * r);
-            kb = kk > kb ? kk : kb;
-        }
-        keys[g] = kb;
+    for (int r = 0; r < 4; ++r) {
+        ar[r] = Mre[(c + 8 * r) * BLD + j];
+        ai[r] = Mim[(c + 8 * r) * BLD + j];
     }
-    __syncthreads();
+    unsigned used = 0u;  // bit i: row i has been a pivot (the same in every thread)
     for (int k = 0; k < BM; ++k) {
-        const double* Sre = (k & 1) ? Wre : Mre;
-        const double* Sim = (k & 1) ? Wim : Mim;
-        double* Dre = (k & 1) ? Mre : Wre;
-        double* Dim = (k & 1) ? Mim : Wim;
-        const unsigned long long* kin = keys + 8 * (k & 1);
-        unsigned long long kb = kin[0];
-#pragma unroll
-        for (int sl = 1; sl < 8; ++sl) {
-            const unsigned long long kk = kin[sl];
-            kb = kk > kb ? kk : kb;
-        }
-        const int p = (kb == 0) ? k : 31 - (int)(kb & 31ull);  // no candidate only for a NaN column
-        const int poff = p * BLD, koff = k * BLD;
-        const double ar = Sre[poff + k], ai = Sim[poff + k];
-        const double rr = Sre[poff + j], ri = Sim[poff + j];
-        double fr[4], fi[4], oR[4], oI[4];
+        double* cre = Wre + 32 * (k & 1);
+        double* cim = Wim + 32 * (k & 1);
+        // pivot candidates of the owners of column k (every lane runs the
+        // shuffles; xor offsets 1, 2, 4 stay inside each 8-lane column group)
+        unsigned long long kb = 0ull;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int so = (g + 8 * r == p) ? koff : roff[r];  // rows k and p trade places
-            fr[r] = Sre[so + k];
-            fi[r] = Sim[so + k];
-            oR[r] = Sre[so + j];
-            oI[r] = Sim[so + j];
+            const int i = c + 8 * r;
+            const unsigned long long kk = ((used >> i) & 1u) ? 0ull : pivot_key(mag2(ar[r], ai[r]), i);
+            kb = kk > kb ? kk : kb;
         }
-        const double inv = 1.0 / mag2(ar, ai);
-        const double pr = ar * inv, pi = -ai * inv;  // 1 / pivot
+        kb = max8(kb);
+        double* pinv = Wre + 64 + 2 * (k & 1);
+        if (j == k) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                cre[c + 8 * r] = ar[r];
+                cim[c + 8 * r] = ai[r];
+            }
+            const int pk = kb == 0ull ? __builtin_ctz(~used) : 31 - (int)(kb & 31ull);
+            if (c == 0) perm[k] = pk;
+            if (c == (pk & 7)) {
+                const int rk = pk >> 3;
+                const double a = rk == 0 ? ar[0] : rk == 1 ? ar[1] : rk == 2 ? ar[2] : ar[3];
+                const double b = rk == 0 ? ai[0] : rk == 1 ? ai[1] : rk == 2 ? ai[2] : ai[3];
+                const double inv = 1.0 / mag2(a, b);
+                pinv[0] = a * inv;
+                pinv[1] = -b * inv;
+            }
+        }
+        __syncthreads();
+        const int p = __builtin_amdgcn_readfirstlane(perm[k]);
+        used |= 1u << p;
+        const double pr = pinv[0], pi = pinv[1];  // 1 / pivot
+        // the pivot row's entry of column j: lane (column group) + p % 8, register p / 8
+        const int rp = p >> 3;
+        const double sr = rp == 0 ? ar[0] : rp == 1 ? ar[1] : rp == 2 ? ar[2] : ar[3];
+        const double si = rp == 0 ? ai[0] : rp == 1 ? ai[1] : rp == 2 ? ai[2] : ai[3];
+        const int src = (lane & ~7) | (p & 7);
+        const double rr = __shfl(sr, src, 64), ri = __shfl(si, src, 64);
         const bool jk = (j == k);
         const double nr = jk ? pr : rr * pr - ri * pi;  // new pivot row, column j
         const double ni = jk ? pi : rr * pi + ri * pr;
-        unsigned long long kc = 0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = g + 8 * r;
-            const double a0 = jk ? 0.0 : oR[r], b0 = jk ? 0.0 : oI[r];
-            const double ur = a0 - (fr[r] * nr - fi[r] * ni);
-            const double ui = b0 - (fr[r] * ni + fi[r] * nr);
-            const double vr = (i == k) ? nr : ur;
-            const double vi = (i == k) ? ni : ui;
-            Dre[roff[r] + j] = vr;
-            Dim[roff[r] + j] = vi;
-            // candidates of the next pivot column (threads of column k + 1)
-            const unsigned long long kk = (i > k) ? pivot_key(mag2(vr, vi), i) : 0ull;
-            kc = kk > kc ? kk : kc;
+            const int i = c + 8 * r;
+            const double fr = cre[i], fi = cim[i];
+            const double a0 = jk ? 0.0 : ar[r], b0 = jk ? 0.0 : ai[r];
+            const double ur = a0 - (fr * nr - fi * ni);
+            const double ui = b0 - (fr * ni + fi * nr);
+            ar[r] = (i == p) ? nr : ur;
+            ai[r] = (i == p) ? ni : ui;
         }
-        if (j == k + 1) keys[8 * ((k + 1) & 1) + g] = kc;
-        if (tid == 0) perm[k] = p;
-        __syncthreads();
     }
-    // undo the row swaps as column swaps (perm[31] first .. perm[0] last):
-    // column j of the result is column c of the present matrix
-    int c = j;
-    for (int k = 0; k < BM; ++k) {
-        const int pk = perm[k];
-        c = (c == k) ? pk : (c == pk ? k : c);
-    }
-    double vr[4], vi[4];
+    // read back through the pivot order
+    int q = 0;
+    for (int k = 0; k < BM; ++k) q = (perm[k] == j) ? k : q;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        vr[r] = Mre[roff[r] + c];
-        vi[r] = Mim[roff[r] + c];
+        Mre[(c + 8 * r) * BLD + j] = ar[r];
+        Mim[(c + 8 * r) * BLD + j] = ai[r];
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        Mre[roff[r] + j] = vr[r];
-        Mim[roff[r] + j] = vi[r];
+        const int src = perm[c + 8 * r] * BLD + q;
+        ar[r] = Mre[src];
+        ai[r] = Mim[src];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        Mre[(c + 8 * r) * BLD + j] = ar[r];
+        Mim[(c + 8 * r) * BLD + j] = ai[r];
     }
     __syncthreads();
 }
@@ -244,7 +273,6 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
                                                                 int* __restrict__ iters)
 {
     __shared__ double Tl[BPL], eps_s[2][BPL], eps[2][BPL], a[2][BPL], b[2][BPL], G[2][BPL], t1[2][BPL];
-    __shared__ unsigned long long keys[16];
     __shared__ int perm[BM];
     __shared__ double wmax[4];
     const int e = blockIdx.x, side = blockIdx.y;
@@ -279,7 +307,7 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
             G[1][i * BLD + j] = (i == j ? Ei : 0.0) - eps[1][i * BLD + j];
         }
         __syncthreads();
-        gj_inverse(G[0], G[1], t1[0], t1[1], keys, perm);
+        gj_inverse(G[0], G[1], t1[0], t1[1], perm);
         // ga = G a, gb = G b
         d4 r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0}, r2 = {0, 0, 0, 0}, i2 = {0, 0, 0, 0};
         mm<true, true, false, false>(r1, i1, G[0], G[1], a[0], a[1], q);
@@ -328,7 +356,7 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
         G[1][i * BLD + j] = (i == j ? Ei : 0.0) - eps_s[1][i * BLD + j];
     }
     __syncthreads();
-    gj_inverse(G[0], G[1], t1[0], t1[1], keys, perm);
+    gj_inverse(G[0], G[1], t1[0], t1[1], perm);
     d4 r1 = {0, 0, 0, 0}, i1 = {0, 0, 0, 0};
     if (side == 0) {
         // Sigma_L = T^T (g T)
@@ -372,7 +400,6 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
                                                              double2* __restrict__ diag)
 {
     __shared__ double Tl[BPL], Hl[BPL], C[3][2][BPL];
-    __shared__ unsigned long long keys[16];
     __shared__ int perm[BM];
     const int el = blockIdx.x;
     const int e = e0 + el;
@@ -425,7 +452,7 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
             Cur[1][i * BLD + j] = ai;
         }
         __syncthreads();
-        gj_inverse(Cur[0], Cur[1], P[0], P[1], keys, perm);
+        gj_inverse(Cur[0], Cur[1], P[0], P[1], perm);
         if (b < nb - 1) store_cplx(gs + (size_t)b * 2 * BGL, Cur[0], Cur[1]);
     }
     // ---- backward sweep: Cur holds G_{nb-1} = g_{nb-1}
@@ -692,14 +719,13 @@ __global__ __launch_bounds__(BTHREADS) void block_probe_kernel(int op, const dou
                                                                const double* __restrict__ Bm, double* __restrict__ Cm)
 {
     __shared__ double a[2][BPL], b[2][BPL];
-    __shared__ unsigned long long keys[16];
     __shared__ int perm[BM];
     const Quad q;
     load_cplx(a[0], a[1], A);
     load_cplx(b[0], b[1], Bm);
     __syncthreads();
     if (op == 1) {
-        gj_inverse(a[0], a[1], b[0], b[1], keys, perm);
+        gj_inverse(a[0], a[1], b[0], b[1], perm);
         store_cplx(Cm, a[0], a[1]);
         return;
     }
