@@ -61,8 +61,10 @@ struct Quad {
 };
 
 // (cre, cim) += A B over this wavefront's quadrant; AC/BC: complex operand;
-// TA: A(i,k) is read at A[k][i] (A stored transposed); TB: B(k,j) at B[j][k].
-template <bool AC, bool BC, bool TA, bool TB>
+// TA: A(i,k) is read at A[k][i] (A stored transposed); TB: B(k,j) at B[j][k];
+// LA, LB: row strides (BLD for LDS planes, BM for a real matrix read from
+// global memory, which stays in the L1/L2 caches).
+template <bool AC, bool BC, bool TA, bool TB, int LA = BLD, int LB = BLD>
 __device__ __forceinline__ void mm(d4& cre, d4& cim, const double* Are, const double* Aim, const double* Bre,
                                    const double* Bim, const Quad& q)
 {
@@ -70,8 +72,8 @@ __device__ __forceinline__ void mm(d4& cre, d4& cim, const double* Are, const do
 #pragma unroll
     for (int k0 = 0; k0 < BM; k0 += 4) {
         const int k = k0 + kk;
-        const int ia = TA ? k * BLD + i : i * BLD + k;
-        const int ib = TB ? j * BLD + k : k * BLD + j;
+        const int ia = TA ? k * LA + i : i * LA + k;
+        const int ib = TB ? j * LB + k : k * LB + j;
         const double ar = Are[ia], br = Bre[ib];
         if (AC && BC) {
             const double ai = Aim[ia], bi = Bim[ib];
@@ -390,7 +392,7 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
 // Block RGF for energy e0 + blockIdx.x: forward sweep of left-connected
 // g_b (stored to gstore[blockIdx.x][b]), backward sweep of G_bb; writes
 // part[e][b*32 + o] = Im(coef_e G_e,bb,oo) and, if diag, G_e,bb,oo.
-__global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int nb, const double2* __restrict__ E,
+__global__ __launch_bounds__(BTHREADS, 3) void block_rgf_kernel(int e0, int nE, int nb, const double2* __restrict__ E,
                                                              const double2* __restrict__ coef,
                                                              const double* __restrict__ H0g,
                                                              const double* __restrict__ Tg,
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
                                                              double* __restrict__ gstore, double* __restrict__ part,
                                                              double2* __restrict__ diag)
 {
-    __shared__ double Tl[BPL], Hl[BPL], C[3][2][BPL];
+    __shared__ double C[3][2][BPL];  // T and H0 are read from global memory (cached)
     __shared__ int perm[BM];
     const int el = blockIdx.x;
     const int e = e0 + el;
@@ -411,9 +413,6 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
     const double* SL = Sigma + (size_t)e * 4 * BGL;
     const double* SR = SL + 2 * BGL;
     double* gs = gstore + (size_t)el * (nb > 1 ? nb - 1 : 1) * 2 * BGL;  // g_0 .. g_{nb-2}
-    load_real(Tl, Tg);
-    load_real(Hl, H0g);
-    __syncthreads();
     double* Cur[2] = {C[0][0], C[0][1]};
     double* P[2] = {C[1][0], C[1][1]};
     // ---- forward sweep
@@ -422,10 +421,10 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
         if (b > 0) {
             // P = g_{b-1} T, S = T^T P
             d4 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
-            mm<true, false, false, false>(pr, pi, Cur[0], Cur[1], Tl, nullptr, q);
+            mm<true, false, false, false, BLD, BM>(pr, pi, Cur[0], Cur[1], Tg, nullptr, q);
             store_quad(P[0], P[1], pr, pi, q);
             __syncthreads();
-            mm<false, true, true, false>(sr, si, Tl, nullptr, P[0], P[1], q);
+            mm<false, true, true, false, BM, BLD>(sr, si, Tg, nullptr, P[0], P[1], q);
             __syncthreads();
         }
         // A = E - H0 - diag(v_b) - S (- Sigma_L on the first block, - Sigma_R on the last)
@@ -434,7 +433,7 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
             const int i = q.row(vv);
-            double ar = -Hl[i * BLD + j] - sr[vv];
+            double ar = -H0g[i * BM + j] - sr[vv];
             double ai = -si[vv];
             if (i == j) {
                 ar += Er - vb[i];
@@ -472,7 +471,7 @@ __global__ __launch_bounds__(BTHREADS) void block_rgf_kernel(int e0, int nE, int
         load_cplx(W[0], W[1], gs + (size_t)b * 2 * BGL);
         __syncthreads();
         d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
-        mm<true, false, false, false>(xr, xi, W[0], W[1], Tl, nullptr, q);  // X = g_b T
+        mm<true, false, false, false, BLD, BM>(xr, xi, W[0], W[1], Tg, nullptr, q);  // X = g_b T
         store_quad(X[0], X[1], xr, xi, q);
         __syncthreads();
         d4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
