@@ -69,7 +69,7 @@ __device__ __forceinline__ void mm(d4& cre, d4& cim, const double* Are, const do
                                    const double* Bim, const Quad& q)
 {
     const int i = q.qi * 16 + (q.lane & 15), j = q.qj * 16 + (q.lane & 15), kk = q.lane >> 4;
-#pragma unroll
+#pragma unroll 2
     for (int k0 = 0; k0 < BM; k0 += 4) {
         const int k = k0 + kk;
         const int ia = TA ? k * LA + i : i * LA + k;
@@ -162,6 +162,7 @@ __device__ void gj_inverse(double* Mre, double* Mim, double* Wre, double* Wim, i
         ai[r] = Mim[(c + 8 * r) * BLD + j];
     }
     unsigned used = 0u;  // bit i: row i has been a pivot (the same in every thread)
+    int q = 0;           // the step whose pivot row is j
     for (int k = 0; k < BM; ++k) {
         double* cre = Wre + 32 * (k & 1);
         double* cim = Wim + 32 * (k & 1);
@@ -196,6 +197,7 @@ __device__ void gj_inverse(double* Mre, double* Mim, double* Wre, double* Wim, i
         __syncthreads();
         const int p = __builtin_amdgcn_readfirstlane(perm[k]);
         used |= 1u << p;
+        q = (p == j) ? k : q;
         const double pr = pinv[0], pi = pinv[1];  // 1 / pivot
         // the pivot row's entry of column j: lane (column group) + p % 8, register p / 8
         const int rp = p >> 3;
@@ -218,8 +220,6 @@ __device__ void gj_inverse(double* Mre, double* Mim, double* Wre, double* Wim, i
         }
     }
     // read back through the pivot order
-    int q = 0;
-    for (int k = 0; k < BM; ++k) q = (perm[k] == j) ? k : q;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         Mre[(c + 8 * r) * BLD + j] = ar[r];
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
 // Block RGF for energy e0 + blockIdx.x: forward sweep of left-connected
 // g_b (stored to gstore[blockIdx.x][b]), backward sweep of G_bb; writes
 // part[e][b*32 + o] = Im(coef_e G_e,bb,oo) and, if diag, G_e,bb,oo.
-__global__ __launch_bounds__(BTHREADS, 3) void block_rgf_kernel(int e0, int nE, int nb, const double2* __restrict__ E,
+__global__ __launch_bounds__(BTHREADS, 4) void block_rgf_kernel(int e0, int nE, int nb, const double2* __restrict__ E,
                                                              const double2* __restrict__ coef,
                                                              const double* __restrict__ H0g,
                                                              const double* __restrict__ Tg,
@@ -401,7 +401,9 @@ __global__ __launch_bounds__(BTHREADS, 3) void block_rgf_kernel(int e0, int nE, 
                                                              double* __restrict__ gstore, double* __restrict__ part,
                                                              double2* __restrict__ diag)
 {
-    __shared__ double C[3][2][BPL];  // T and H0 are read from global memory (cached)
+    // two complex LDS planes (33.8 KB, so four workgroups share a CU); T, H0
+    // and the stored g_b are read from global memory through the caches
+    __shared__ double C[2][2][BPL];
     __shared__ int perm[BM];
     const int el = blockIdx.x;
     const int e = e0 + el;
@@ -465,13 +467,11 @@ __global__ __launch_bounds__(BTHREADS, 3) void block_rgf_kernel(int e0, int nE, 
     emit(nb - 1, Cur[0], Cur[1]);
     double* Gn[2] = {Cur[0], Cur[1]};
     double* X[2] = {C[1][0], C[1][1]};
-    double* W[2] = {C[2][0], C[2][1]};
+    const int j = q.col();
     for (int b = nb - 2; b >= 0; --b) {
-        __syncthreads();
-        load_cplx(W[0], W[1], gs + (size_t)b * 2 * BGL);
-        __syncthreads();
+        const double* gb = gs + (size_t)b * 2 * BGL;
         d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
-        mm<true, false, false, false, BLD, BM>(xr, xi, W[0], W[1], Tg, nullptr, q);  // X = g_b T
+        mm<true, false, false, false, BM, BM>(xr, xi, gb, gb + BGL, Tg, nullptr, q);  // X = g_b T
         store_quad(X[0], X[1], xr, xi, q);
         __syncthreads();
         d4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
@@ -479,24 +479,17 @@ __global__ __launch_bounds__(BTHREADS, 3) void block_rgf_kernel(int e0, int nE, 
         __syncthreads();
         store_quad(Gn[0], Gn[1], yr, yi, q);  // Gn <- Y
         __syncthreads();
-        d4 gr = {0, 0, 0, 0}, gi = {0, 0, 0, 0};
-        mm<true, true, false, true>(gr, gi, Gn[0], Gn[1], X[0], X[1], q);  // Y X^T
-        const int j = q.col();
+        d4 gr, gi;
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
-            const int o = q.row(vv) * BLD + j;
-            W[0][o] += gr[vv];
-            W[1][o] += gi[vv];
+            gr[vv] = gb[q.row(vv) * BM + j];
+            gi[vv] = gb[BGL + q.row(vv) * BM + j];
         }
+        mm<true, true, false, true>(gr, gi, Gn[0], Gn[1], X[0], X[1], q);  // G_b = g_b + Y X^T
         __syncthreads();
-        emit(b, W[0], W[1]);
-        // rotate: G_b becomes the next G_{b+1}; the old Gn buffer is free
-        double* t0 = Gn[0];
-        double* t1 = Gn[1];
-        Gn[0] = W[0];
-        Gn[1] = W[1];
-        W[0] = t0;
-        W[1] = t1;
+        store_quad(Gn[0], Gn[1], gr, gi, q);
+        __syncthreads();
+        emit(b, Gn[0], Gn[1]);
     }
 }
 
