@@ -262,6 +262,240 @@ __device__ __forceinline__ void store_cplx(double* __restrict__ dst, const doubl
     }
 }
 
+// ---------------------------------------------------------------- one
+// wavefront per energy.  The block RGF sweep gives each energy a wavefront
+// of its own: the whole 32 x 32 product runs on that wavefront's MFMA
+// (four 16 x 16 quadrants, index qi * 2 + qj; result register v of lane l
+// holds row qi * 16 + l / 16 + 4 v, column qj * 16 + l % 16), and the
+// inverse keeps the matrix in its registers.  Lanes of one wavefront exchange
+// data through LDS in program order, so no workgroup barrier is needed.
+constexpr int WPB = 4;  // energies (wavefronts) per workgroup
+
+struct Quad4 {
+    d4 re[4], im[4];
+};
+
+__device__ __forceinline__ void zero(Quad4& c)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        c.re[q] = d4{0, 0, 0, 0};
+        c.im[q] = d4{0, 0, 0, 0};
+    }
+}
+
+// LDS written by lanes of this wavefront is visible to its later reads.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// c += A B over all four quadrants (same operand conventions as mm).
+template <bool AC, bool BC, bool TA, bool TB, int LA = BLD, int LB = BLD>
+__device__ __forceinline__ void wmm(Quad4& c, const double* Are, const double* Aim, const double* Bre,
+                                    const double* Bim, int lane)
+{
+    const int il = lane & 15, kk = lane >> 4;
+#pragma unroll 2
+    for (int k0 = 0; k0 < BM; k0 += 4) {
+        const int k = k0 + kk;
+        double ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int i = h * 16 + il;
+            const int ia = TA ? k * LA + i : i * LA + k;
+            const int ib = TB ? i * LB + k : k * LB + i;
+            ar[h] = Are[ia];
+            ai[h] = AC ? Aim[ia] : 0.0;
+            br[h] = Bre[ib];
+            bi[h] = BC ? Bim[ib] : 0.0;
+        }
+#pragma unroll
+        for (int qi = 0; qi < 2; ++qi)
+#pragma unroll
+            for (int qj = 0; qj < 2; ++qj) {
+                const int q = qi * 2 + qj;
+                if (AC && BC) {
+                    c.re[q] = mfma64(ar[qi], br[qj], c.re[q]);
+                    c.re[q] = mfma64(-ai[qi], bi[qj], c.re[q]);
+                    c.im[q] = mfma64(ar[qi], bi[qj], c.im[q]);
+                    c.im[q] = mfma64(ai[qi], br[qj], c.im[q]);
+                } else if (AC) {
+                    c.re[q] = mfma64(ar[qi], br[qj], c.re[q]);
+                    c.im[q] = mfma64(ai[qi], br[qj], c.im[q]);
+                } else {
+                    c.re[q] = mfma64(ar[qi], br[qj], c.re[q]);
+                    c.im[q] = mfma64(ar[qi], bi[qj], c.im[q]);
+                }
+            }
+    }
+}
+
+__device__ __forceinline__ int wrow(int lane, int qi, int v) { return qi * 16 + (lane >> 4) + 4 * v; }
+__device__ __forceinline__ int wcol(int lane, int qj) { return qj * 16 + (lane & 15); }
+
+__device__ __forceinline__ void store_quad4(double* Mre, double* Mim, const Quad4& c, int lane)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int o = wrow(lane, q >> 1, v) * BLD + wcol(lane, q & 1);
+            Mre[o] = c.re[q][v];
+            Mim[o] = c.im[q][v];
+        }
+}
+
+// 32-bit pivot key: |a|^2 rounded to float (non-negative, so its bits order
+// as an unsigned integer) with the low 5 mantissa bits replaced by 31 - row:
+// the largest key is a largest magnitude (to float precision, enough for a
+// pivot choice) and, among equal ones, the smallest row.  0: no candidate.
+__device__ __forceinline__ unsigned pivot_key32(double m, int i)
+{
+    return (__float_as_uint((float)m) & ~31u) | (unsigned)(31 - i);
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned max_dpp(unsigned v)
+{
+    const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+    return o > v ? o : v;
+}
+
+// max over the 64 lanes, read from lane 63: quad_perm [1,0,3,2], quad_perm
+// [2,3,0,1], row_half_mirror and row_mirror give every lane its row's
+// maximum; row_bcast:15 and row_bcast:31 carry the rows into lane 63.
+__device__ __forceinline__ unsigned wave_max32(unsigned v)
+{
+    v = max_dpp<0xB1>(v);
+    v = max_dpp<0x4E>(v);
+    v = max_dpp<0x141>(v);
+    v = max_dpp<0x140>(v);
+    v = max_dpp<0x142>(v);
+    v = max_dpp<0x143>(v);
+    return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+__device__ __forceinline__ double dpp_swap1(double x)
+{
+    return __longlong_as_double((long long)dpp64<0xB1>((unsigned long long)__double_as_longlong(x)));
+}
+
+__device__ __forceinline__ double readlane_f64(double x, int src)
+{
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = __builtin_amdgcn_readlane((unsigned)(b & 0xffffffffll), src);
+    const unsigned hi = __builtin_amdgcn_readlane((unsigned)((unsigned long long)b >> 32), src);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// 1 / d for d > 0: hardware reciprocal estimate refined by two Newton-Raphson
+// steps y <- y + y (1 - d y), each of which squares the relative error.
+__device__ __forceinline__ double recip_nr(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(y, fma(-d, y, 1.0), y);
+    return fma(y, fma(-d, y, 1.0), y);
+}
+
+// Inverse of the complex matrix (Mre, Mim) (LDS planes, row stride BLD) by
+// one wavefront, in place: Gauss-Jordan with partial pivoting.  Lane l holds
+// row i = l / 2, columns h * 16 .. h * 16 + 15 (h = l % 2) in registers.
+// Rows are never swapped: step k takes as pivot the largest |a_ik| among the
+// rows not yet used (|a_ik|^2 compared in float precision, ties to the
+// smallest row), found by a 64-lane maximum.
+// A pivot row is not divided by its pivot: each row keeps a scale s_i (1 until
+// it is the pivot, then 1 / pivot) and holds M / s_i, which leaves every
+// update in the form a_ij -= (a_ik / pivot) a_pj with the stored values; the
+// scales are applied once at the end.  The pivot element and the pivot row's
+// entries in column k + 1 are read from their lanes directly, so the next
+// pivot search waits on no memory; the rest of the pivot row reaches the
+// other lanes through rb (two alternating buffers of 32 complex entries in
+// LDS).  perm (32 ints of LDS) records the pivot order.  The result is
+// written back through the pivot order: A^-1[q(i)][perm[j]] = s_i M[i][j],
+// with q(i) the step whose pivot was row i.
+__device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, double2* rb, int* perm, int lane)
+{
+    const int i = lane >> 1, h = lane & 1;
+    double ar[16], ai[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        ar[r] = Mre[i * BLD + h * 16 + r];
+        ai[r] = Mim[i * BLD + h * 16 + r];
+    }
+    unsigned used = 0u;  // bit x: row x has been a pivot (wave-uniform)
+    int qrow = 0;
+    double sr = 1.0, si = 0.0;
+    for (int kh = 0; kh < 2; ++kh) {
+#pragma unroll
+        for (int kr = 0; kr < 16; ++kr) {
+            const int k = kh * 16 + kr;
+            const int c = (kr + 1) & 15;  // register of column k + 1 in the lanes of its half
+            double2* rbk = rb + (kr & 1) * BM;
+            // pivot: column k is register kr of the lanes with h == kh
+            unsigned key = 0u;
+            if (h == kh && !((used >> i) & 1u)) key = pivot_key32(mag2(ar[kr], ai[kr]), i);
+            key = wave_max32(key);
+            const int p = key == 0u ? __builtin_ctz(~used) : 31 - (int)(key & 31u);
+            used |= 1u << p;
+            if (i == p) {
+#pragma unroll
+                for (int r = 0; r < 16; ++r) rbk[h * 16 + r] = make_double2(ar[r], ai[r]);
+                if (h == 0) perm[k] = p;
+                qrow = k;
+            }
+            // 1 / pivot from the lane holding (p, k)
+            const double pr = readlane_f64(ar[kr], 2 * p + kh), pim = readlane_f64(ai[kr], 2 * p + kh);
+            const double iv = recip_nr(pr * pr + pim * pim);
+            const double ir = pr * iv, ii = -pim * iv;
+            // multiplier of this row: a_ik / pivot (a_ik from the lane of this row holding column k)
+            const double orr = dpp_swap1(ar[kr]), oi = dpp_swap1(ai[kr]);
+            const double fr = (h == kh) ? ar[kr] : orr, fi = (h == kh) ? ai[kr] : oi;
+            double mr = fr * ir - fi * ii, mi = fr * ii + fi * ir;
+            if (i == p) {
+                mr = 0.0;
+                mi = 0.0;
+                sr = ir;
+                si = ii;
+            }
+            // register c first, with the pivot row's entries read from its two lanes
+            {
+                const double x0r = readlane_f64(ar[c], 2 * p), x0i = readlane_f64(ai[c], 2 * p);
+                const double x1r = readlane_f64(ar[c], 2 * p + 1), x1i = readlane_f64(ai[c], 2 * p + 1);
+                const double xr = h ? x1r : x0r, xi = h ? x1i : x0i;
+                ar[c] = fma(-mr, xr, fma(mi, xi, ar[c]));
+                ai[c] = fma(-mr, xi, fma(-mi, xr, ai[c]));
+            }
+            wave_sync();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                if (r == c) continue;
+                const double2 x = rbk[h * 16 + r];
+                ar[r] = fma(-mr, x.x, fma(mi, x.y, ar[r]));
+                ai[r] = fma(-mr, x.y, fma(-mi, x.x, ai[r]));
+            }
+            if (h == kh) {
+                ar[kr] = (i == p) ? 1.0 : -mr;
+                ai[kr] = (i == p) ? 0.0 : -mi;
+            }
+        }
+    }
+    wave_sync();
+    int pj[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) pj[r] = perm[h * 16 + r];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int o = qrow * BLD + pj[r];
+        Mre[o] = sr * ar[r] - si * ai[r];
+        Mim[o] = sr * ai[r] + si * ar[r];
+    }
+    wave_sync();
+}
+
 // Sancho-Rubio decimation for one energy (blockIdx.x) and one lead
 // (blockIdx.y: 0 = left lead, surface = its last block, alpha = T^T;
 // 1 = right lead, surface = its first block, alpha = T), on-site H0 + Uc.
@@ -389,10 +623,12 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
     if (tid == 0) iters[e * 2 + side] = conv ? it : -it;
 }
 
-// Block RGF for energy e0 + blockIdx.x: forward sweep of left-connected
-// g_b (stored to gstore[blockIdx.x][b]), backward sweep of G_bb; writes
-// part[e][b*32 + o] = Im(coef_e G_e,bb,oo) and, if diag, G_e,bb,oo.
-__global__ __launch_bounds__(BTHREADS, 4) void block_rgf_kernel(int e0, int nE, int nb, const double2* __restrict__ E,
+// Block RGF, one wavefront per energy: energy e0 + el with el = blockIdx.x *
+// WPB + wavefront (el < n).  Forward sweep of left-connected g_b (stored to
+// gstore[el][b]), backward sweep of G_bb; writes part[e][b*32 + o] =
+// Im(coef_e G_e,bb,oo) and, if diag, G_e,bb,oo.  T is shared in LDS by the
+// workgroup's wavefronts; H0 and the stored g_b are read through the caches.
+__global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, int nb, const double2* __restrict__ E,
                                                              const double2* __restrict__ coef,
                                                              const double* __restrict__ H0g,
                                                              const double* __restrict__ Tg,
@@ -401,95 +637,107 @@ __global__ __launch_bounds__(BTHREADS, 4) void block_rgf_kernel(int e0, int nE, 
                                                              double* __restrict__ gstore, double* __restrict__ part,
                                                              double2* __restrict__ diag)
 {
-    // two complex LDS planes (33.8 KB, so four workgroups share a CU); T, H0
-    // and the stored g_b are read from global memory through the caches
-    __shared__ double C[2][2][BPL];
-    __shared__ int perm[BM];
-    const int el = blockIdx.x;
+    __shared__ double Ts[BPL];
+    __shared__ double pl[WPB][4][BPL];  // per wavefront: two complex planes
+    __shared__ double2 rbuf[WPB][2 * BM];
+    __shared__ int perm_s[WPB][BM];
+    load_real(Ts, Tg);
+    __syncthreads();  // the only workgroup barrier
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int el = blockIdx.x * WPB + w;
+    if (el >= n) return;  // whole wavefront
     const int e = e0 + el;
-    if (e >= nE) return;
-    const int tid = threadIdx.x;
-    const Quad q;
     const double Er = E[e].x, Ei = E[e].y;
     const double cr = coef[e].x, ci = coef[e].y;
     const double* SL = Sigma + (size_t)e * 4 * BGL;
     const double* SR = SL + 2 * BGL;
     double* gs = gstore + (size_t)el * (nb > 1 ? nb - 1 : 1) * 2 * BGL;  // g_0 .. g_{nb-2}
-    double* Cur[2] = {C[0][0], C[0][1]};
-    double* P[2] = {C[1][0], C[1][1]};
+    double* Cre = pl[w][0];
+    double* Cim = pl[w][1];
+    double* Pre = pl[w][2];
+    double* Pim = pl[w][3];
     // ---- forward sweep
     for (int b = 0; b < nb; ++b) {
-        d4 sr = {0, 0, 0, 0}, si = {0, 0, 0, 0};
+        Quad4 S;
+        zero(S);
         if (b > 0) {
-            // P = g_{b-1} T, S = T^T P
-            d4 pr = {0, 0, 0, 0}, pi = {0, 0, 0, 0};
-            mm<true, false, false, false, BLD, BM>(pr, pi, Cur[0], Cur[1], Tg, nullptr, q);
-            store_quad(P[0], P[1], pr, pi, q);
-            __syncthreads();
-            mm<false, true, true, false, BM, BLD>(sr, si, Tg, nullptr, P[0], P[1], q);
-            __syncthreads();
+            Quad4 P;
+            zero(P);
+            wmm<true, false, false, false>(P, Cre, Cim, Ts, nullptr, lane);  // g_{b-1} T
+            store_quad4(Pre, Pim, P, lane);
+            wave_sync();
+            wmm<false, true, true, false>(S, Ts, nullptr, Pre, Pim, lane);  // T^T g_{b-1} T
         }
         // A = E - H0 - diag(v_b) - S (- Sigma_L on the first block, - Sigma_R on the last)
         const double* vb = v + (size_t)b * BM;
-        const int j = q.col();
 #pragma unroll
-        for (int vv = 0; vv < 4; ++vv) {
-            const int i = q.row(vv);
-            double ar = -H0g[i * BM + j] - sr[vv];
-            double ai = -si[vv];
-            if (i == j) {
-                ar += Er - vb[i];
-                ai += Ei;
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int i = wrow(lane, q >> 1, vv), j = wcol(lane, q & 1);
+                double ar = -H0g[i * BM + j] - S.re[q][vv];
+                double ai = -S.im[q][vv];
+                if (i == j) {
+                    ar += Er - vb[i];
+                    ai += Ei;
+                }
+                if (b == 0) {
+                    ar -= SL[i * BM + j];
+                    ai -= SL[BGL + i * BM + j];
+                }
+                if (b == nb - 1) {
+                    ar -= SR[i * BM + j];
+                    ai -= SR[BGL + i * BM + j];
+                }
+                Cre[i * BLD + j] = ar;
+                Cim[i * BLD + j] = ai;
             }
-            if (b == 0) {
-                ar -= SL[i * BM + j];
-                ai -= SL[BGL + i * BM + j];
+        wave_sync();
+        gj_inverse_wave(Cre, Cim, rbuf[w], perm_s[w], lane);
+        if (b < nb - 1) {
+            double* dst = gs + (size_t)b * 2 * BGL;
+            for (int x = lane; x < BGL; x += 64) {
+                dst[x] = Cre[(x >> 5) * BLD + (x & 31)];
+                dst[BGL + x] = Cim[(x >> 5) * BLD + (x & 31)];
             }
-            if (b == nb - 1) {
-                ar -= SR[i * BM + j];
-                ai -= SR[BGL + i * BM + j];
-            }
-            Cur[0][i * BLD + j] = ar;
-            Cur[1][i * BLD + j] = ai;
         }
-        __syncthreads();
-        gj_inverse(Cur[0], Cur[1], P[0], P[1], perm);
-        if (b < nb - 1) store_cplx(gs + (size_t)b * 2 * BGL, Cur[0], Cur[1]);
     }
-    // ---- backward sweep: Cur holds G_{nb-1} = g_{nb-1}
-    auto emit = [&](int b, const double* Gre, const double* Gim) {
-        if (tid < BM) {
-            const double gr = Gre[tid * BLD + tid], gi = Gim[tid * BLD + tid];
-            part[(size_t)e * nb * BM + (size_t)b * BM + tid] = cr * gi + ci * gr;
-            if (diag) diag[(size_t)e * nb * BM + (size_t)b * BM + tid] = make_double2(gr, gi);
+    // ---- backward sweep: C holds G_{nb-1} = g_{nb-1}
+    auto emit = [&](int b) {
+        if (lane < BM) {
+            const double gr = Cre[lane * BLD + lane], gi = Cim[lane * BLD + lane];
+            part[(size_t)e * nb * BM + (size_t)b * BM + lane] = cr * gi + ci * gr;
+            if (diag) diag[(size_t)e * nb * BM + (size_t)b * BM + lane] = make_double2(gr, gi);
         }
     };
-    emit(nb - 1, Cur[0], Cur[1]);
-    double* Gn[2] = {Cur[0], Cur[1]};
-    double* X[2] = {C[1][0], C[1][1]};
-    const int j = q.col();
+    emit(nb - 1);
     for (int b = nb - 2; b >= 0; --b) {
         const double* gb = gs + (size_t)b * 2 * BGL;
-        d4 xr = {0, 0, 0, 0}, xi = {0, 0, 0, 0};
-        mm<true, false, false, false, BM, BM>(xr, xi, gb, gb + BGL, Tg, nullptr, q);  // X = g_b T
-        store_quad(X[0], X[1], xr, xi, q);
-        __syncthreads();
-        d4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
-        mm<true, true, false, false>(yr, yi, X[0], X[1], Gn[0], Gn[1], q);  // Y = X G_{b+1}
-        __syncthreads();
-        store_quad(Gn[0], Gn[1], yr, yi, q);  // Gn <- Y
-        __syncthreads();
-        d4 gr, gi;
+        Quad4 X;
+        zero(X);
+        wmm<true, false, false, false, BM, BLD>(X, gb, gb + BGL, Ts, nullptr, lane);  // X = g_b T
+        store_quad4(Pre, Pim, X, lane);
+        wave_sync();
+        Quad4 Y;
+        zero(Y);
+        wmm<true, true, false, false>(Y, Pre, Pim, Cre, Cim, lane);  // Y = X G_{b+1}
+        wave_sync();
+        store_quad4(Cre, Cim, Y, lane);
+        wave_sync();
+        Quad4 G;
 #pragma unroll
-        for (int vv = 0; vv < 4; ++vv) {
-            gr[vv] = gb[q.row(vv) * BM + j];
-            gi[vv] = gb[BGL + q.row(vv) * BM + j];
-        }
-        mm<true, true, false, true>(gr, gi, Gn[0], Gn[1], X[0], X[1], q);  // G_b = g_b + Y X^T
-        __syncthreads();
-        store_quad(Gn[0], Gn[1], gr, gi, q);
-        __syncthreads();
-        emit(b, Gn[0], Gn[1]);
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int vv = 0; vv < 4; ++vv) {
+                const int o = wrow(lane, q >> 1, vv) * BM + wcol(lane, q & 1);
+                G.re[q][vv] = gb[o];
+                G.im[q][vv] = gb[BGL + o];
+            }
+        wmm<true, true, false, true>(G, Cre, Cim, Pre, Pim, lane);  // G_b = g_b + Y X^T
+        wave_sync();
+        store_quad4(Cre, Cim, G, lane);
+        wave_sync();
+        emit(b);
     }
 }
 
@@ -631,7 +879,8 @@ int negf_block_rgf(negf_block_chain* h, int nE, const negf_complex* E, const neg
     NEGF_HIP_TRY(hipEventRecord(c.ev[1], c.st));
     for (int e0 = 0; e0 < nE; e0 += chunk) {
         const int n = std::min(chunk, nE - e0);
-        hipLaunchKernelGGL(block_rgf_kernel, dim3(n), dim3(BTHREADS), 0, c.st, e0, nE, nb, c.d_E.as<double2>(),
+        hipLaunchKernelGGL(block_rgf_kernel, dim3((n + WPB - 1) / WPB), dim3(64 * WPB), 0, c.st, e0, n, nb,
+                           c.d_E.as<double2>(),
                            c.d_coef.as<double2>(), c.d_H0.as<double>(), c.d_T.as<double>(), c.d_v.as<double>(),
                            c.d_Sigma.as<double>(), c.d_g.as<double>(), c.d_part.as<double>(),
                            diagG ? c.d_diag.as<double2>() : nullptr);
@@ -706,16 +955,38 @@ int negf_block_last_time(const negf_block_chain* h, float* ms_sancho_rubio, floa
 
 namespace negf {
 // op 0: C = A B (complex), op 1: C = A^-1 (complex), op 2: C = A B with
-// B real (B im plane ignored); A, B, C: [re plane, im plane] of 32 x 32.
+// B real (B im plane ignored); ops 3, 4, 5: the same by one wavefront (the
+// block RGF sweep's forms).  A, B, C: [re plane, im plane] of 32 x 32.
 __global__ __launch_bounds__(BTHREADS) void block_probe_kernel(int op, const double* __restrict__ A,
                                                                const double* __restrict__ Bm, double* __restrict__ Cm)
 {
     __shared__ double a[2][BPL], b[2][BPL];
     __shared__ int perm[BM];
+    __shared__ double2 rb[2 * BM];
     const Quad q;
     load_cplx(a[0], a[1], A);
     load_cplx(b[0], b[1], Bm);
     __syncthreads();
+    if (op >= 3) {
+        if (threadIdx.x < 64) {
+            const int lane = threadIdx.x;
+            if (op == 4) {
+                gj_inverse_wave(a[0], a[1], rb, perm, lane);
+            } else {
+                Quad4 c;
+                zero(c);
+                if (op == 3)
+                    wmm<true, true, false, false>(c, a[0], a[1], b[0], b[1], lane);
+                else
+                    wmm<true, false, false, false>(c, a[0], a[1], b[0], nullptr, lane);
+                wave_sync();
+                store_quad4(a[0], a[1], c, lane);
+            }
+        }
+        __syncthreads();
+        store_cplx(Cm, a[0], a[1]);
+        return;
+    }
     if (op == 1) {
         gj_inverse(a[0], a[1], b[0], b[1], perm);
         store_cplx(Cm, a[0], a[1]);
@@ -736,7 +1007,7 @@ __global__ __launch_bounds__(BTHREADS) void block_probe_kernel(int op, const dou
 extern "C" int negf_block_probe(int op, const double* A, const double* B, double* C)
 {
     negf::clear_error();
-    if (op < 0 || op > 2 || !A || !B || !C) {
+    if (op < 0 || op > 5 || !A || !B || !C) {
         negf::set_error("invalid probe arguments");
         return NEGF_EARG;
     }

@@ -350,7 +350,9 @@ int negf_block_self_energies(negf_block_chain *c, int nE, const negf_complex *E,
                              double sr_tol, int sr_max_iter, double *sigma, int *sr_iters);
 /* Self-check of the device building blocks on one 32 x 32 complex matrix
  * pair (re plane then im plane, row-major): op 0: C = A B, op 1: C = A^-1
- * (pivoted Gauss-Jordan), op 2: C = A Re(B).  No reference counterpart. */
+ * (pivoted Gauss-Jordan), op 2: C = A Re(B), each by a workgroup of four
+ * wavefronts; ops 3, 4, 5: the same three by one wavefront (the forms the
+ * block RGF sweep uses).  No reference counterpart. */
 int negf_block_probe(int op, const double *A, const double *B, double *C);
 /* Device time (ms) of the decimation and of the RGF sweep of the last call. */
 int negf_block_last_time(const negf_block_chain *c, float *ms_sancho_rubio, float *ms_rgf);

@@ -104,10 +104,11 @@ def test_block_rgf_rejects_bad_arguments(dev):
     c.close()
 
 
-@pytest.mark.parametrize("op", [0, 1, 2])
+@pytest.mark.parametrize("op", [0, 1, 2, 3, 4, 5])
 def test_block_building_blocks(dev, op):
     # the MFMA quadrant products (operand and result lane layout) and the
-    # pivoted Gauss-Jordan inverse on one matrix pair, against numpy
+    # pivoted Gauss-Jordan inverse on one matrix pair, against numpy; ops 0-2
+    # by a workgroup of four wavefronts, ops 3-5 by a single wavefront
     import ctypes as C
     from electrostatic_amd import _lib
     rng = np.random.default_rng(op)
@@ -120,5 +121,5 @@ def test_block_building_blocks(dev, op):
     _lib.check(_lib.load().negf_block_probe(op, a.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p),
                                             out.ctypes.data_as(C.c_void_p)))
     Cm = out[:1024].reshape(32, 32) + 1j * out[1024:].reshape(32, 32)
-    ref = {0: A @ Bm, 1: np.linalg.inv(A), 2: A @ Bm.real}[op]
+    ref = {0: A @ Bm, 1: np.linalg.inv(A), 2: A @ Bm.real}[op % 3]
     assert rel(Cm, ref) < 1e-12
