@@ -241,7 +241,9 @@ def block_leg(steps=3, with_cpu=True):
     real axis [-3, 3] eV with eta = 1e-5 eV), on-site disorder N(0, 0.05 eV)
     per orbital (seed 4) plus a smooth potential; Sancho-Rubio leads to
     1e-12.  Algorithmic flops per energy-block: 36 m^3 (forward: g T, T^T (gT),
-    inverse; backward: g T, X G, Y X^T; real H0 and T)."""
+    inverse; backward: g T, X G, Y X^T; real H0 and T).  The general-block
+    count of the same sweep (complex T, a second inverse for G: 56 m^3) is
+    reported beside it."""
     from electrostatic_amd import negf
     from oracle import block_oracle as B
     nb, nE, m = 2000, 1024, 32
@@ -273,7 +275,9 @@ def block_leg(steps=3, with_cpu=True):
            "sancho_rubio_iterations_max": int(it.max()),
            "energy_blocks_per_s": nb * nE / (wall * 1e-3),
            "rgf_tflops": flops / (rg * 1e-3) / 1e12,
-           "rgf_frac_fp64_peak": flops / (rg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
+           "rgf_frac_fp64_peak": flops / (rg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+           "rgf_frac_fp64_peak_general_block_count": 56 * m ** 3 * nb * nE / (rg * 1e-3) / 1e12
+                                                     / FP64_PEAK_TFLOPS}
     if with_cpu:
         # numpy oracle on 4 energies of the same chain
         sample = E[::nE // 4][:4]
