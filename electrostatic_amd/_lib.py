@@ -135,6 +135,7 @@ SIGNATURES = {
     "negf_block_rgf": (C.c_int, [vp, C.c_int, vp, vp, C.c_double, C.c_int, dp, vp, ip]),
     "negf_block_self_energies": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_int, dp, ip]),
     "negf_block_probe": (C.c_int, [C.c_int, vp, vp, vp]),
+    "negf_block_chain_set_scratch_limit": (C.c_int, [vp, C.c_size_t]),
     "negf_block_last_time": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "negf_batch_scf_get_kernel": (C.c_int, [vp, dp]),
     "negf_batch_scf_set_preconditioner": (C.c_int, [vp, dp, dp]),

@@ -943,6 +943,17 @@ int negf_block_self_energies(negf_block_chain* h, int nE, const negf_complex* E,
     return NEGF_OK;
 }
 
+int negf_block_chain_set_scratch_limit(negf_block_chain* h, size_t bytes)
+{
+    clear_error();
+    if (!h || bytes == 0) {
+        set_error("invalid scratch limit");
+        return NEGF_EARG;
+    }
+    h->c.scratch_limit = bytes;
+    return NEGF_OK;
+}
+
 int negf_block_last_time(const negf_block_chain* h, float* ms_sancho_rubio, float* ms_rgf)
 {
     if (!h) return NEGF_EARG;

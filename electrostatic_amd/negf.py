@@ -482,6 +482,11 @@ class BlockChain:
         uc = np.ascontiguousarray(Uc, np.float64)
         _lib.check(self.lib.negf_block_chain_set_potential(self.h, _dp(v), _dp(uc)))
 
+    def set_scratch_limit(self, nbytes):
+        """Bound on the device scratch for the stored g_b (the sweep runs the
+        energies in chunks that fit; results do not depend on it)."""
+        _lib.check(self.lib.negf_block_chain_set_scratch_limit(self.h, int(nbytes)))
+
     def rgf(self, E, coef=None, tol=1e-12, max_iter=200, want_diag=False):
         """(acc [nb, 32], diagG [nE, nb, 32] or None, Sancho-Rubio iterations [nE, 2])."""
         E = np.ascontiguousarray(np.atleast_1d(E), np.complex128)

@@ -31,6 +31,19 @@ def test_library_exports_every_declared_symbol():
     assert set(names) <= set(_lib.SIGNATURES), set(names) - set(_lib.SIGNATURES)
 
 
+def test_header_is_plain_c(tmp_path):
+    # a C caller (cgo, a C material class) includes the header as C99
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "use.c"
+    src.write_text('#include "negf_amd.h"\nint main(void) { return negf_block_probe == 0; }\n')
+    subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.dirname(HEADER), str(src)],
+                   check=True)
+
+
 def test_version_and_limits():
     lib = _lib.load()
     assert lib.negf_version() >= 100

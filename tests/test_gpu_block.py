@@ -94,6 +94,28 @@ def test_block_rgf_ring_traces_match_mode_space(dev):
     c.close()
 
 
+def test_block_rgf_energy_chunks_are_bit_identical(dev):
+    # a scratch limit of two energies' g_b splits 7 energies into sweeps of
+    # 2, 2, 2, 1 (the last workgroup partly empty); every energy is computed
+    # independently and the charge is summed in energy order, so the result
+    # equals the single sweep bit for bit
+    nb = 9
+    H0, T, v, Uc = chain(nb, 5)
+    E = np.array([0.37 + 1e-5j, -0.8 + 1e-5j, 1.9 + 1e-3j, -2.6 + 0.05j, 0.02 + 1e-5j, 0.9 + 1e-2j,
+                  -1.4 + 1e-5j])
+    coef = np.exp(1j * np.linspace(0.0, 1.0, E.size))
+    c = negf.BlockChain(nb, H0, T)
+    c.set_potential(v, Uc)
+    acc1, d1, _ = c.rgf(E, coef, want_diag=True)
+    c.set_scratch_limit(2 * (nb - 1) * 2 * 32 * 32 * 8)
+    acc2, d2, _ = c.rgf(E, coef, want_diag=True)
+    with pytest.raises(Exception):
+        c.set_scratch_limit(0)
+    c.close()
+    assert np.array_equal(acc1, acc2)
+    assert np.array_equal(d1, d2)
+
+
 def test_block_rgf_rejects_bad_arguments(dev):
     H0, T = B.zigzag_blocks(16)
     with pytest.raises(Exception):
