@@ -39,7 +39,8 @@ def test_header_is_plain_c(tmp_path):
     if cc is None:
         pytest.skip("no C compiler")
     src = tmp_path / "use.c"
-    src.write_text('#include "negf_amd.h"\nint main(void) { return negf_block_probe == 0; }\n')
+    src.write_text('#include "negf_amd.h"\nint (*probe)(int, const double *, const double *, double *) = '
+                   'negf_block_probe;\nint main(void) { return 0; }\n')
     subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.dirname(HEADER), str(src)],
                    check=True)
 
