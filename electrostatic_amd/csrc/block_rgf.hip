@@ -293,8 +293,9 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// c += A B over all four quadrants (same operand conventions as mm).
-template <bool AC, bool BC, bool TA, bool TB, int LA = BLD, int LB = BLD>
+// c += A B over all four quadrants (same operand conventions as mm); SYM:
+// the product is known to be symmetric and quadrant (1, 0) is skipped.
+template <bool AC, bool BC, bool TA, bool TB, int LA = BLD, int LB = BLD, bool SYM = false>
 __device__ __forceinline__ void wmm(Quad4& c, const double* Are, const double* Aim, const double* Bre,
                                     const double* Bim, int lane)
 {
@@ -318,6 +319,7 @@ __device__ __forceinline__ void wmm(Quad4& c, const double* Are, const double* A
 #pragma unroll
             for (int qj = 0; qj < 2; ++qj) {
                 const int q = qi * 2 + qj;
+                if (SYM && q == 2) continue;
                 if (AC && BC) {
                     c.re[q] = mfma64(ar[qi], br[qj], c.re[q]);
                     c.re[q] = mfma64(-ai[qi], bi[qj], c.re[q]);
@@ -347,6 +349,26 @@ __device__ __forceinline__ void store_quad4(double* Mre, double* Mim, const Quad
             Mre[o] = c.re[q][v];
             Mim[o] = c.im[q][v];
         }
+}
+
+// store of a symmetric matrix of which quadrants 0, 1 and 3 are held:
+// quadrant 1 is written to both of its mirror positions.
+__device__ __forceinline__ void store_quad4_sym(double* Mre, double* Mim, const Quad4& c, int lane)
+{
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (q == 2) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int i = wrow(lane, q >> 1, v), j = wcol(lane, q & 1);
+            Mre[i * BLD + j] = c.re[q][v];
+            Mim[i * BLD + j] = c.im[q][v];
+            if (q == 1) {
+                Mre[j * BLD + i] = c.re[q][v];
+                Mim[j * BLD + i] = c.im[q][v];
+            }
+        }
+    }
 }
 
 // 32-bit pivot key: |a|^2 rounded to float (non-negative, so its bits order
@@ -666,12 +688,14 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
             wmm<true, false, false, false>(P, Cre, Cim, Ts, nullptr, lane);  // g_{b-1} T
             store_quad4(Pre, Pim, P, lane);
             wave_sync();
-            wmm<false, true, true, false>(S, Ts, nullptr, Pre, Pim, lane);  // T^T g_{b-1} T
+            wmm<false, true, true, false, BLD, BLD, true>(S, Ts, nullptr, Pre, Pim, lane);  // T^T g_{b-1} T
         }
-        // A = E - H0 - diag(v_b) - S (- Sigma_L on the first block, - Sigma_R on the last)
+        // A = E - H0 - diag(v_b) - S (- Sigma_L on the first block, - Sigma_R on the last);
+        // A is complex symmetric: quadrant (1, 0) is the mirror of quadrant (0, 1)
         const double* vb = v + (size_t)b * BM;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 4; ++q) {
+            if (q == 2) continue;
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
                 const int i = wrow(lane, q >> 1, vv), j = wcol(lane, q & 1);
@@ -691,7 +715,12 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
                 }
                 Cre[i * BLD + j] = ar;
                 Cim[i * BLD + j] = ai;
+                if (q == 1) {
+                    Cre[j * BLD + i] = ar;
+                    Cim[j * BLD + i] = ai;
+                }
             }
+        }
         wave_sync();
         gj_inverse_wave(Cre, Cim, rbuf[w], perm_s[w], lane);
         if (b < nb - 1) {
@@ -724,18 +753,21 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
         wave_sync();
         store_quad4(Cre, Cim, Y, lane);
         wave_sync();
-        Quad4 G;
+        Quad4 G;  // G_b is complex symmetric: quadrants 0, 1, 3
+        zero(G);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < 4; ++q) {
+            if (q == 2) continue;
 #pragma unroll
             for (int vv = 0; vv < 4; ++vv) {
                 const int o = wrow(lane, q >> 1, vv) * BM + wcol(lane, q & 1);
                 G.re[q][vv] = gb[o];
                 G.im[q][vv] = gb[BGL + o];
             }
-        wmm<true, true, false, true>(G, Cre, Cim, Pre, Pim, lane);  // G_b = g_b + Y X^T
+        }
+        wmm<true, true, false, true, BLD, BLD, true>(G, Cre, Cim, Pre, Pim, lane);  // G_b = g_b + Y X^T
         wave_sync();
-        store_quad4(Cre, Cim, G, lane);
+        store_quad4_sym(Cre, Cim, G, lane);
         wave_sync();
         emit(b);
     }
