@@ -15,15 +15,16 @@
 // functions are then complex symmetric, which the backward step uses:
 //   G_bb = g_b + X G_{b+1,b+1} X^T,   X = g_b T.
 //
-// Layout: one workgroup (4 wavefronts) per energy; each wavefront owns one
-// 16 x 16 quadrant of every 32 x 32 product and runs it on
-// v_mfma_f64_16x16x4f64 (lane l: A[i][k] with i = l % 16, k = l / 16;
-// B[k][j] with j = l % 16; result register v holds D[l / 16 + 4 v][l % 16],
-// as a GPU probe against numpy established for this FP64 instruction).  Complex
-// matrices are two real planes in LDS (row stride 33 doubles).  Inverses are
-// in-place Gauss-Jordan with partial pivoting over the whole workgroup.  The
-// forward sweep stores each g_b (16 KiB) to a device scratch array that the
-// backward sweep reads back.
+// Layout.  Products run on v_mfma_f64_16x16x4f64 (lane l: A[i][k] with
+// i = l % 16, k = l / 16; B[k][j] with j = l % 16; result register v holds
+// D[l / 16 + 4 v][l % 16], as a GPU probe against numpy established for this
+// FP64 instruction).  Complex matrices are two real planes in LDS (row stride
+// 33 doubles).  The block RGF sweep gives each energy one wavefront, which
+// runs all four 16 x 16 quadrants of every product and keeps the matrix of the
+// pivoted Gauss-Jordan inverse in its registers; the Sancho-Rubio kernel gives
+// each (energy, lead) a workgroup of four wavefronts, one quadrant each, with
+// the inverse spread over the workgroup.  The forward sweep stores each g_b
+// (16 KiB) to a device scratch array that the backward sweep reads back.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +40,7 @@ typedef __attribute__((ext_vector_type(4))) double d4;
 constexpr int BM = 32;               // orbitals per block
 constexpr int BLD = 33;              // LDS row stride (doubles)
 constexpr int BPL = BM * BLD;        // doubles per LDS plane
-constexpr int BTHREADS = 256;        // 4 wavefronts, one quadrant each
+constexpr int BTHREADS = 256;        // 4 wavefronts (Sancho-Rubio: one quadrant each)
 constexpr int BGL = BM * BM;         // doubles per plane in global memory (row-major, unpadded)
 
 __device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
