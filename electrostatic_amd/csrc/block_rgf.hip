@@ -271,6 +271,7 @@ __device__ __forceinline__ void store_cplx(double* __restrict__ dst, const doubl
 // inverse keeps the matrix in its registers.  Lanes of one wavefront exchange
 // data through LDS in program order, so no workgroup barrier is needed.
 constexpr int WPB = 4;  // energies (wavefronts) per workgroup
+static_assert(64 * WPB == BTHREADS, "the workgroup-wide loads stride by BTHREADS");
 
 struct Quad4 {
     d4 re[4], im[4];
