@@ -435,13 +435,14 @@ __device__ __forceinline__ double recip_nr(double d)
 // it is the pivot, then 1 / pivot) and holds M / s_i, which leaves every
 // update in the form a_ij -= (a_ik / pivot) a_pj with the stored values; the
 // scales are applied once at the end.  The pivot element and the pivot row's
-// entries in column k + 1 are read from their lanes directly, so the next
-// pivot search waits on no memory; the rest of the pivot row reaches the
-// other lanes through rb (two alternating buffers of 32 complex entries in
-// LDS).  perm (32 ints of LDS) records the pivot order.  The result is
-// written back through the pivot order: A^-1[q(i)][perm[j]] = s_i M[i][j],
-// with q(i) the step whose pivot was row i.
-__device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, double2* rb, int* perm, int lane)
+// entries in column k + 1 are read from their lanes (v_readlane), so the next
+// pivot search waits on no memory; every lane fetches the pivot row's other
+// entries of its columns from the pivot row's lane of the same half by lane
+// shuffles (ds_bpermute), so no LDS store, wait and reload sits in a step.
+// perm (32 ints of LDS) records the pivot order.  The result is written back
+// through the pivot order: A^-1[q(i)][perm[j]] = s_i M[i][j], with q(i) the
+// step whose pivot was row i.
+__device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, int* perm, int lane)
 {
     const int i = lane >> 1, h = lane & 1;
     double ar[16], ai[16];
@@ -457,8 +458,6 @@ __device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, double
 #pragma unroll
         for (int kr = 0; kr < 16; ++kr) {
             const int k = kh * 16 + kr;
-            const int c = (kr + 1) & 15;  // register of column k + 1 in the lanes of its half
-            double2* rbk = rb + (kr & 1) * BM;
             // pivot: column k is register kr of the lanes with h == kh
             unsigned key = 0u;
             if (h == kh && !((used >> i) & 1u)) key = pivot_key32(mag2(ar[kr], ai[kr]), i);
@@ -466,8 +465,6 @@ __device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, double
             const int p = key == 0u ? __builtin_ctz(~used) : 31 - (int)(key & 31u);
             used |= 1u << p;
             if (i == p) {
-#pragma unroll
-                for (int r = 0; r < 16; ++r) rbk[h * 16 + r] = make_double2(ar[r], ai[r]);
                 if (h == 0) perm[k] = p;
                 qrow = k;
             }
@@ -485,7 +482,10 @@ __device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, double
                 sr = ir;
                 si = ii;
             }
-            // register c first, with the pivot row's entries read from its two lanes
+            // register c (column k + 1 in the lanes of its half) first, with the
+            // pivot row's entries read from its two lanes: the next pivot search
+            // then waits on no shuffle
+            const int c = (kr + 1) & 15;
             {
                 const double x0r = readlane_f64(ar[c], 2 * p), x0i = readlane_f64(ai[c], 2 * p);
                 const double x1r = readlane_f64(ar[c], 2 * p + 1), x1i = readlane_f64(ai[c], 2 * p + 1);
@@ -493,11 +493,13 @@ __device__ __forceinline__ void gj_inverse_wave(double* Mre, double* Mim, double
                 ar[c] = fma(-mr, xr, fma(mi, xi, ar[c]));
                 ai[c] = fma(-mr, xi, fma(-mi, xr, ai[c]));
             }
-            wave_sync();
+            // the other entries of the pivot row, from the lane of its same half
+            // (the pivot row itself is not updated: m = 0)
+            const int src = 2 * p + h;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 if (r == c) continue;
-                const double2 x = rbk[h * 16 + r];
+                const double2 x = make_double2(__shfl(ar[r], src, 64), __shfl(ai[r], src, 64));
                 ar[r] = fma(-mr, x.x, fma(mi, x.y, ar[r]));
                 ai[r] = fma(-mr, x.y, fma(-mi, x.x, ai[r]));
             }
@@ -663,7 +665,6 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
 {
     __shared__ double Ts[BPL];
     __shared__ double pl[WPB][4][BPL];  // per wavefront: two complex planes
-    __shared__ double2 rbuf[WPB][2 * BM];
     __shared__ int perm_s[WPB][BM];
     load_real(Ts, Tg);
     __syncthreads();  // the only workgroup barrier
@@ -724,7 +725,7 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
             }
         }
         wave_sync();
-        gj_inverse_wave(Cre, Cim, rbuf[w], perm_s[w], lane);
+        gj_inverse_wave(Cre, Cim, perm_s[w], lane);
         if (b < nb - 1) {
             double* dst = gs + (size_t)b * 2 * BGL;
             for (int x = lane; x < BGL; x += 64) {
@@ -1007,7 +1008,6 @@ __global__ __launch_bounds__(BTHREADS) void block_probe_kernel(int op, const dou
 {
     __shared__ double a[2][BPL], b[2][BPL];
     __shared__ int perm[BM];
-    __shared__ double2 rb[2 * BM];
     const Quad q;
     load_cplx(a[0], a[1], A);
     load_cplx(b[0], b[1], Bm);
@@ -1016,7 +1016,7 @@ __global__ __launch_bounds__(BTHREADS) void block_probe_kernel(int op, const dou
         if (threadIdx.x < 64) {
             const int lane = threadIdx.x;
             if (op == 4) {
-                gj_inverse_wave(a[0], a[1], rb, perm, lane);
+                gj_inverse_wave(a[0], a[1], perm, lane);
             } else {
                 Quad4 c;
                 zero(c);
