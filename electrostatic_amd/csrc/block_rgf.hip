@@ -654,6 +654,9 @@ __global__ __launch_bounds__(BTHREADS) void sancho_rubio_kernel(int nE, const do
 // gstore[el][b]), backward sweep of G_bb; writes part[e][b*32 + o] =
 // Im(coef_e G_e,bb,oo) and, if diag, G_e,bb,oo.  T is shared in LDS by the
 // workgroup's wavefronts; H0 and the stored g_b are read through the caches.
+// KEEP_X: the forward product g_b T (computed for S) is stored beside g_b and
+// read back as the backward's X instead of being computed again.
+template <bool KEEP_X>
 __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, int nb, const double2* __restrict__ E,
                                                              const double2* __restrict__ coef,
                                                              const double* __restrict__ H0g,
@@ -676,7 +679,9 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
     const double cr = coef[e].x, ci = coef[e].y;
     const double* SL = Sigma + (size_t)e * 4 * BGL;
     const double* SR = SL + 2 * BGL;
-    double* gs = gstore + (size_t)el * (nb > 1 ? nb - 1 : 1) * 2 * BGL;  // g_0 .. g_{nb-2}
+    const size_t nstore = nb > 1 ? nb - 1 : 1;
+    double* gs = gstore + (size_t)el * nstore * (KEEP_X ? 2 : 1) * 2 * BGL;  // g_0 .. g_{nb-2}
+    double* xs = gs + nstore * 2 * BGL;                                      // (KEEP_X) X_0 .. X_{nb-2}
     double* Cre = pl[w][0];
     double* Cim = pl[w][1];
     double* Pre = pl[w][2];
@@ -690,6 +695,17 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
             zero(P);
             wmm<true, false, false, false>(P, Cre, Cim, Ts, nullptr, lane);  // g_{b-1} T
             store_quad4(Pre, Pim, P, lane);
+            if (KEEP_X) {
+                double* dst = xs + (size_t)(b - 1) * 2 * BGL;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+#pragma unroll
+                    for (int vv = 0; vv < 4; ++vv) {
+                        const int o = wrow(lane, q >> 1, vv) * BM + wcol(lane, q & 1);
+                        dst[o] = P.re[q][vv];
+                        dst[BGL + o] = P.im[q][vv];
+                    }
+            }
             wave_sync();
             wmm<false, true, true, false, BLD, BLD, true>(S, Ts, nullptr, Pre, Pim, lane);  // T^T g_{b-1} T
         }
@@ -746,8 +762,20 @@ __global__ __launch_bounds__(64 * WPB, 1) void block_rgf_kernel(int e0, int n, i
     for (int b = nb - 2; b >= 0; --b) {
         const double* gb = gs + (size_t)b * 2 * BGL;
         Quad4 X;
-        zero(X);
-        wmm<true, false, false, false, BM, BLD>(X, gb, gb + BGL, Ts, nullptr, lane);  // X = g_b T
+        if (KEEP_X) {
+            const double* src = xs + (size_t)b * 2 * BGL;  // X = g_b T from the forward sweep
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int vv = 0; vv < 4; ++vv) {
+                    const int o = wrow(lane, q >> 1, vv) * BM + wcol(lane, q & 1);
+                    X.re[q][vv] = src[o];
+                    X.im[q][vv] = src[BGL + o];
+                }
+        } else {
+            zero(X);
+            wmm<true, false, false, false, BM, BLD>(X, gb, gb + BGL, Ts, nullptr, lane);  // X = g_b T
+        }
         store_quad4(Pre, Pim, X, lane);
         wave_sync();
         Quad4 Y;
@@ -791,7 +819,7 @@ struct BlockChain {
     std::vector<double> H0, T;
     double Uc[2] = {0.0, 0.0};
     DevBuf d_H0, d_T, d_v, d_E, d_coef, d_Sigma, d_iters, d_g, d_part, d_out, d_diag;
-    size_t scratch_limit = (size_t)40 << 30;  // bytes of g_b storage per launch
+    size_t scratch_limit = (size_t)80 << 30;  // bytes of g_b (and g_b T) storage per launch
     float ms_sr = 0.f, ms_rgf = 0.f;
     hipStream_t st = nullptr;
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -900,7 +928,10 @@ int negf_block_rgf(negf_block_chain* h, int nE, const negf_complex* E, const neg
     if ((rc = c.d_part.ensure(sizeof(double) * (size_t)nE * len))) return rc;
     if ((rc = c.d_out.ensure(sizeof(double) * len))) return rc;
     if (diagG && (rc = c.d_diag.ensure(sizeof(double2) * (size_t)nE * len))) return rc;
-    const size_t per_energy = (size_t)std::max(nb - 1, 1) * 2 * BGL * sizeof(double);
+    // g_b for every block and energy; also g_b T when all energies fit in one sweep that way
+    const size_t per_g = (size_t)std::max(nb - 1, 1) * 2 * BGL * sizeof(double);
+    const bool keep_x = 2 * per_g * (size_t)nE <= c.scratch_limit;
+    const size_t per_energy = keep_x ? 2 * per_g : per_g;
     const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)nE, c.scratch_limit / per_energy));
     if ((rc = c.d_g.ensure(per_energy * chunk))) return rc;
     NEGF_HIP_TRY(hipMemcpyAsync(c.d_E.p, E, sizeof(double2) * nE, hipMemcpyHostToDevice, c.st));
@@ -914,11 +945,16 @@ int negf_block_rgf(negf_block_chain* h, int nE, const negf_complex* E, const neg
     NEGF_HIP_TRY(hipEventRecord(c.ev[1], c.st));
     for (int e0 = 0; e0 < nE; e0 += chunk) {
         const int n = std::min(chunk, nE - e0);
-        hipLaunchKernelGGL(block_rgf_kernel, dim3((n + WPB - 1) / WPB), dim3(64 * WPB), 0, c.st, e0, n, nb,
-                           c.d_E.as<double2>(),
-                           c.d_coef.as<double2>(), c.d_H0.as<double>(), c.d_T.as<double>(), c.d_v.as<double>(),
-                           c.d_Sigma.as<double>(), c.d_g.as<double>(), c.d_part.as<double>(),
-                           diagG ? c.d_diag.as<double2>() : nullptr);
+        if (keep_x)
+            hipLaunchKernelGGL(block_rgf_kernel<true>, dim3((n + WPB - 1) / WPB), dim3(64 * WPB), 0, c.st, e0, n,
+                               nb, c.d_E.as<double2>(), c.d_coef.as<double2>(), c.d_H0.as<double>(),
+                               c.d_T.as<double>(), c.d_v.as<double>(), c.d_Sigma.as<double>(), c.d_g.as<double>(),
+                               c.d_part.as<double>(), diagG ? c.d_diag.as<double2>() : nullptr);
+        else
+            hipLaunchKernelGGL(block_rgf_kernel<false>, dim3((n + WPB - 1) / WPB), dim3(64 * WPB), 0, c.st, e0, n,
+                               nb, c.d_E.as<double2>(), c.d_coef.as<double2>(), c.d_H0.as<double>(),
+                               c.d_T.as<double>(), c.d_v.as<double>(), c.d_Sigma.as<double>(), c.d_g.as<double>(),
+                               c.d_part.as<double>(), diagG ? c.d_diag.as<double2>() : nullptr);
         NEGF_HIP_TRY(hipGetLastError());
     }
     NEGF_HIP_TRY(hipEventRecord(c.ev[2], c.st));

@@ -359,9 +359,11 @@ int negf_block_probe(int op, const double *A, const double *B, double *C);
 /* Device time (ms) of the decimation and of the RGF sweep of the last call. */
 int negf_block_last_time(const negf_block_chain *c, float *ms_sancho_rubio, float *ms_rgf);
 /* Upper bound (bytes) on the device scratch that holds the stored g_b
- * (16 KiB per block and energy); negf_block_rgf runs the energies in as many
- * sweeps as needed to stay below it (default 40 GiB; at least one energy per
- * sweep).  Results do not depend on it.  No reference counterpart. */
+ * (16 KiB per block and energy) and, when all energies fit in one sweep that
+ * way, the stored products g_b T (another 16 KiB); negf_block_rgf runs the
+ * energies in as many sweeps as needed to stay below it (default 80 GiB; at
+ * least one energy per sweep).  Results do not depend on it.  No reference
+ * counterpart. */
 int negf_block_chain_set_scratch_limit(negf_block_chain *c, size_t bytes);
 
 /* ------------------------------------------------------------------------
