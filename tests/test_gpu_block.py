@@ -41,12 +41,17 @@ def test_sancho_rubio_self_energies_match_oracle(dev):
     c.close()
 
 
+@pytest.mark.parametrize("keep_x", [True, False])
 @pytest.mark.parametrize("nb", [1, 2, 7])
-def test_block_rgf_diag_matches_oracle(dev, nb):
+def test_block_rgf_diag_matches_oracle(dev, nb, keep_x):
+    # keep_x False: a scratch bound of one g_b set per energy makes the
+    # backward sweep recompute g_b T instead of reading it back
     H0, T, v, Uc = chain(nb, seed=nb)
     E = np.array([0.37 + 1e-5j, -0.8 + 1e-5j, 1.9 + 1e-3j, -2.6 + 0.05j, 0.02 + 1e-5j])
     c = negf.BlockChain(nb, H0, T)
     c.set_potential(v, Uc)
+    if not keep_x:
+        c.set_scratch_limit(E.size * max(nb - 1, 1) * 2 * 32 * 32 * 8)
     acc, dg, it = c.rgf(E, want_diag=True)
     assert np.all(it > 0)
     ref_acc = np.zeros((nb, 32))
