@@ -324,12 +324,17 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     lp.sblk_chain = d_sblk_chain_.as<int>();
     lp.status = d_status_.as<int>();
     lp.kind = kind_;
+    lp.dec_use = dec_.use;
+    lp.dec_layers = dec_.layers;
+    lp.dec_max_iter = dec_.max_iter;
+    lp.dec_rel_error = dec_.rel_error;
     lp.tube_active = tube_active;
 
-    hipLaunchKernelGGL(sigma_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, d_chains_,
-                       d_chain_tube_.as<int>(), nchains_, d_tubes_.as<WTube>(), dec_.use, dec_.layers,
-                       dec_.max_iter, dec_.rel_error, d_status_.as<int>());
-    NEGF_HIP_TRY(hipGetLastError());
+    if (dec_.use) {
+        hipLaunchKernelGGL(sigma_dec_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, lp,
+                           d_chain_tube_.as<int>(), nchains_);
+        NEGF_HIP_TRY(hipGetLastError());
+    }
     if (timing_) {
         if (ev_used_ == ev_start_.size()) {
             hipEvent_t a, b;

@@ -78,7 +78,7 @@ class RgfPlan {
     // (chains of tube t are [tubes[t].chain0, +nchain)); Sigma is written into it.
     int build(int kind, bool write_G, const std::vector<RgfTube>& tubes, ChainDesc* d_chains, int nchains,
               const DecParams& dec);
-    // Enqueue: lead self-energies, transfer matrices, segment scan, sweeps and
+    // Enqueue: transfer matrices (with the lead self-energies), segment scan, sweeps and
     // the fixed-order row reduction.  d_out: per-tube outputs at out_off
     // (KIND_EQ/NONEQ); d_diagG: [chain][N]; d_chain_out: per-chain outputs
     // (integrand samples [slot][3], DOS/T pairs [slot][2]).

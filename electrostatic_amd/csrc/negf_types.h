@@ -28,7 +28,7 @@ struct ChainDesc {
     double iL, iR;       // KIND_NONEQ integrand-sample coefficients
     double t_even;       // hopping on even links (mode beta)
     double degen;        // weight used by KIND_DOS
-    double SLr, SLi;     // lead self-energies (filled by sigma_kernel)
+    double SLr, SLi;     // lead self-energies (filled by the transfer launch)
     double SRr, SRi;
     int tube;            // index into the tube table
     int out_slot;        // row of per-chain outputs (integrand samples, DOS)

@@ -77,6 +77,7 @@ struct WLaunch {
     const int* item_list;     // sweep launches: items of this launch (or null: all)
     int nlist;
     const int* tube_active;   // optional [ext]: 0 skips the tube's work (its outputs stay as they were)
+    int dec_use, dec_layers, dec_max_iter, dec_rel_error;  // lead surface GF by decimation (else closed form)
 };
 
 __device__ __forceinline__ bool tube_skipped(const WLaunch& lp, const WTube& tube)
@@ -227,11 +228,48 @@ __device__ __forceinline__ void stage_segment(double* us, const double* __restri
     __builtin_amdgcn_wave_barrier();
 }
 
-// (256, 8): 8 workgroups of 4 waves per CU, i.e. 8 waves per SIMD, which
-// keeps the kernel at <= 64 VGPRs so a 128-group x 64-segment launch runs in
-// one round.
+// Lead self-energies of one chain, Sigma = tau g tau^+ with tau = t_odd
+// (get_Sigma_at_contacts NEGF_Common.cpp:4165-4177, CNT.cpp:348-351), written
+// into its ChainDesc for the scan and the sweeps.  DEC: the surface GF by
+// decimation (CNT.cpp:358-361) instead of the closed form.
+template <bool DEC>
+__device__ __forceinline__ void lead_sigma(const WLaunch& lp, ChainDesc* out, const ChainDesc& ch, const WTube& tube)
+{
+    const double2 E = make_double2(ch.Er, ch.Ei);
+    const double te = ch.t_even, to = tube.t_odd;
+    const double Uc0 = tube.Ucp ? tube.Ucp[0] : tube.Uc0;
+    const double Uc1 = tube.Ucp ? tube.Ucp[1] : tube.Uc1;
+    double2 gL, gR;
+    int okL = 1, okR = 1;
+    if (DEC) {
+        gL = decimation_gf(make_double2(E.x - Uc0, E.y), te, to, lp.dec_layers, lp.dec_max_iter, lp.dec_rel_error);
+        gR = decimation_gf(make_double2(E.x - Uc1, E.y), te, to, lp.dec_layers, lp.dec_max_iter, lp.dec_rel_error);
+    } else {
+        gL = surface_gf(E, Uc0, te, to, &okL);
+        gR = surface_gf(E, Uc1, te, to, &okR);
+    }
+    if (!okL || !okR) atomicOr(lp.status, 1);
+    out->SLr = to * to * gL.x;
+    out->SLi = to * to * gL.y;
+    out->SRr = to * to * gR.x;
+    out->SRi = to * to * gR.y;
+}
+
+// Lead self-energies by decimation, one thread per chain, before the transfer
+// launch (the decimation's layer arrays live in scratch, which the transfer
+// kernel does not carry).
+__global__ __launch_bounds__(256) void sigma_dec_kernel(WLaunch lp, const int* chain_tube, int nchains)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nchains) return;
+    const WTube tube = lp.tubes[chain_tube[c]];
+    lead_sigma<true>(lp, lp.chains + c, lp.chains[c], tube);
+}
+
+// (256, 4): up to 4 workgroups of 4 waves per CU, enough registers for the
+// lead self-energies without spills (4, 6 or 8 waves per SIMD time the same).
 template <int K>
-__global__ __launch_bounds__(256, 8) void transfer_kernel(WLaunch lp)
+__global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
 {
     __shared__ double us_all[4][NEGF_SEG_MAX];
     const int lane = threadIdx.x & 63;
@@ -242,6 +280,9 @@ __global__ __launch_bounds__(256, 8) void transfer_kernel(WLaunch lp)
     ItemCtx ic = item_ctx(lp, item, lane, tube);
     if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
+    // the first segment's wavefront also computes its chains' lead self-energies
+    // in closed form (only the scan and the sweeps, later launches, read them)
+    if (!lp.dec_use && ic.s == 0 && ic.valid) lead_sigma<false>(lp, lp.chains + ic.chain, ch, tube);
     const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int lo = min(tube.N, ic.s * tube.L);
     const int hi = min(tube.N, lo + tube.L);
@@ -937,37 +978,6 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_neq_full_kernel(WL
         const double r = lane_sum_transpose<K>(red, lane);
         if (lane < K) prow[j * K + lane] = r;
     }
-}
-
-// Lead self-energies of every chain, Sigma = tau g tau^+ with tau = t_odd
-// (get_Sigma_at_contacts NEGF_Common.cpp:4165-4177, CNT.cpp:348-351); one
-// thread per chain.
-__global__ __launch_bounds__(256) void sigma_kernel(ChainDesc* chains, const int* chain_tube, int nchains,
-                                                    const WTube* tubes, int use_decimation, int dec_layers,
-                                                    int dec_max_iter, int dec_rel_error, int* status)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= nchains) return;
-    const WTube tube = tubes[chain_tube[c]];
-    ChainDesc* ch = chains + c;
-    const double2 E = make_double2(ch->Er, ch->Ei);
-    const double te = ch->t_even, to = tube.t_odd;
-    const double Uc0 = tube.Ucp ? tube.Ucp[0] : tube.Uc0;
-    const double Uc1 = tube.Ucp ? tube.Ucp[1] : tube.Uc1;
-    double2 gL, gR;
-    int okL = 1, okR = 1;
-    if (use_decimation) {
-        gL = decimation_gf(make_double2(E.x - Uc0, E.y), te, to, dec_layers, dec_max_iter, dec_rel_error);
-        gR = decimation_gf(make_double2(E.x - Uc1, E.y), te, to, dec_layers, dec_max_iter, dec_rel_error);
-    } else {
-        gL = surface_gf(E, Uc0, te, to, &okL);
-        gR = surface_gf(E, Uc1, te, to, &okR);
-    }
-    if (!okL || !okR) atomicOr(status, 1);
-    ch->SLr = to * to * gL.x;
-    ch->SLi = to * to * gL.y;
-    ch->SRr = to * to * gR.x;
-    ch->SRi = to * to * gR.y;
 }
 
 // Sum partial rows in a fixed order.  Row r of group t starts at
