@@ -183,6 +183,131 @@ __device__ __forceinline__ void mat_step(const Coefs& k, double U, double2& m00,
     m01 = n1;
 }
 
+// ---- scaled recurrence
+// Dividing the three-term relation t_{n-1} y_{n-1} - alpha_n y_n + t_n y_{n+1}
+// = 0 site by site by Im E: with y_n = s_n w_n and s_{n+1} = s_n ai_n,
+// ai_n = Im E / t_n (forward) and the mirrored scale for the backward
+// direction, both directions become
+//   w_{n+1}  = (a'_n + i) w_n  - tau_{p(n-1)} w_{n-1}
+//   w'_{n-1} = (a'_n + i) w'_n - tau_{p(n)}   w'_{n+1}
+// with a'_n = (Re E - U_n) / Im E and tau_p = (t_p / Im E)^2: the imaginary
+// part of the diagonal coefficient is exactly 1, so a complex step is four
+// FMAs (the unscaled step needs six).  In a segment [lo, hi) with the local
+// scales s_lo = 1 (forward) and s'_{hi-1} = 1 (backward), the product
+// s_n s'_n is the same at every site: C = ai_lo ... ai_{hi-2}, so
+// u_n v_n = C w_n w'_n.  The scales shrink w by |Im E| / t per site against
+// u, so the scaled form is used only when t / |Im E| <= 2^24 (at most 2^384
+// per 16-site renormalisation block).
+struct SCoefs {
+    double ie, er;  // 1 / Im E, Re E / Im E
+    double tau[2];  // (t_p / Im E)^2
+    double ai[2];   // Im E / t_p (index with static parities or selects: a
+                    // run-time index makes the compiler move the whole
+                    // struct to LDS, which slowed the transfer kernel by 40%)
+};
+
+__device__ __forceinline__ SCoefs make_scoefs(double2 E, double te, double to)
+{
+    SCoefs k;
+    k.ie = 1.0 / E.y;
+    k.er = E.x * k.ie;
+    const double a = te * k.ie, b = to * k.ie;
+    k.tau[0] = a * a;
+    k.tau[1] = b * b;
+    k.ai[0] = E.y / te;
+    k.ai[1] = E.y / to;
+    return k;
+}
+
+__device__ __forceinline__ bool scaled_ok(double2 E, double te, double to)
+{
+    return fabs(E.y) * 16777216.0 >= fmax(fabs(te), fabs(to));
+}
+
+// forward scaled step at site n (parity P): (x, y) = (w_n, w_{n-1}) -> (w_{n+1}, w_n)
+template <int P>
+__device__ __forceinline__ void fwd_s(const SCoefs& k, double U, double2& x, double2& y)
+{
+    const double ar = fma(-U, k.ie, k.er);
+    const double tc = k.tau[1 - P];
+    double2 nx;
+    nx.x = fma(ar, x.x, fma(-tc, y.x, -x.y));
+    nx.y = fma(ar, x.y, fma(-tc, y.y, x.x));
+    y = x;
+    x = nx;
+}
+
+// backward scaled step at site n (parity P): (x, y) = (w'_n, w'_{n+1}) -> (w'_{n-1}, w'_n)
+template <int P>
+__device__ __forceinline__ void bwd_s(const SCoefs& k, double U, double2& x, double2& y)
+{
+    const double ar = fma(-U, k.ie, k.er);
+    const double tc = k.tau[P];
+    double2 nx;
+    nx.x = fma(ar, x.x, fma(-tc, y.x, -x.y));
+    nx.y = fma(ar, x.y, fma(-tc, y.y, x.x));
+    y = x;
+    x = nx;
+}
+
+// scaled forward step on both basis columns
+template <int P>
+__device__ __forceinline__ void mat_s(const SCoefs& k, double U, double2& m00, double2& m01, double2& m10,
+                                      double2& m11)
+{
+    const double ar = fma(-U, k.ie, k.er);
+    const double tc = k.tau[1 - P];
+    double2 n0, n1;
+    n0.x = fma(ar, m00.x, fma(-tc, m10.x, -m00.y));
+    n0.y = fma(ar, m00.y, fma(-tc, m10.y, m00.x));
+    n1.x = fma(ar, m01.x, fma(-tc, m11.x, -m01.y));
+    n1.y = fma(ar, m01.y, fma(-tc, m11.y, m01.x));
+    m10 = m00;
+    m11 = m01;
+    m00 = n0;
+    m01 = n1;
+}
+
+// Double-double product (h, l) *= (bh, bl) (Dekker; the FMA gives the exact
+// product error).  Contraction is off so the error terms stay exact.
+__device__ __forceinline__ void dd_mul(double& h, double& l, double bh, double bl)
+{
+#pragma clang fp contract(off)
+    const double p = h * bh;
+    double e = fma(h, bh, -p);
+    e = fma(h, bl, e);
+    e = fma(l, bh, e);
+    const double s = p + e;
+    l = e - (s - p);
+    h = s;
+}
+
+// Product of the scale factors ai over cnt >= 0 consecutive sites whose
+// first site has parity p0, as m 2^e with |m| in [0.5, 1): the pair product
+// ai_0 ai_1 is kept exactly as a double-double and raised to cnt/2 by binary
+// powering, so the result is within about one ulp of the exact product of
+// the rounded factors (a plain power would amplify the rounding of the pair
+// product cnt/2 times).  cnt is wave-uniform.
+__device__ __forceinline__ void scale_prod(const SCoefs& k, int p0, int cnt, double& m, int& e)
+{
+    const int pairs = cnt >> 1;
+    double ph = k.ai[0] * k.ai[1];
+    double pl = fma(k.ai[0], k.ai[1], -ph);
+    const int k0 = exp_of(ph);
+    ph = __builtin_amdgcn_ldexp(ph, -k0);
+    pl = __builtin_amdgcn_ldexp(pl, -k0);
+    double rh = 1.0, rl = 0.0;
+    for (int n = pairs; n; n >>= 1) {
+        if (n & 1) dd_mul(rh, rl, ph, pl);
+        if (n > 1) dd_mul(ph, pl, ph, pl);
+    }
+    if (cnt & 1) dd_mul(rh, rl, p0 ? k.ai[1] : k.ai[0], 0.0);  // the last site has parity p0
+    const double r = rh + rl;
+    const int k1 = exp_of(r);
+    m = __builtin_amdgcn_ldexp(r, -k1);
+    e = k0 * pairs + k1;
+}
+
 __device__ __forceinline__ void mat_mul(const double2 a00, const double2 a01, const double2 a10, const double2 a11,
                                         const double2 b00, const double2 b01, const double2 b10, const double2 b11,
                                         double2& r00, double2& r01, double2& r10, double2& r11)
@@ -266,6 +391,64 @@ __global__ __launch_bounds__(256) void sigma_dec_kernel(WLaunch lp, const int* c
     lead_sigma<true>(lp, lp.chains + c, lp.chains[c], tube);
 }
 
+// Step overloads: plain (Coefs) or scaled (SCoefs) recurrence.
+template <int P>
+__device__ __forceinline__ void mstep(const Coefs& k, double U, double2& a, double2& b, double2& c, double2& d)
+{
+    mat_step<P>(k, U, a, b, c, d);
+}
+template <int P>
+__device__ __forceinline__ void mstep(const SCoefs& k, double U, double2& a, double2& b, double2& c, double2& d)
+{
+    mat_s<P>(k, U, a, b, c, d);
+}
+template <int P>
+__device__ __forceinline__ void fstep(const Coefs& k, double U, double2& x, double2& y)
+{
+    fwd_step<P>(k, U, x, y);
+}
+template <int P>
+__device__ __forceinline__ void fstep(const SCoefs& k, double U, double2& x, double2& y)
+{
+    fwd_s<P>(k, U, x, y);
+}
+template <int P>
+__device__ __forceinline__ void bstep(const Coefs& k, double U, double2& x, double2& y)
+{
+    bwd_step<P>(k, U, x, y);
+}
+template <int P>
+__device__ __forceinline__ void bstep(const SCoefs& k, double U, double2& x, double2& y)
+{
+    bwd_s<P>(k, U, x, y);
+}
+
+// Both basis columns through n sites of the staged segment (the first at an
+// even site), renormalised every K sites.
+template <int K, class KC>
+__device__ __forceinline__ void transfer_loop(const KC& k, const double* us, int n, double2& m00, double2& m01,
+                                              double2& m10, double2& m11, int& eT)
+{
+    for (int q0 = 0; q0 < n; q0 += K) {
+        const int len = min(K, n - q0);
+        if (len == K) {
+#pragma unroll
+            for (int i = 0; i < K; i += 2) {
+                mstep<0>(k, us[q0 + i], m00, m01, m10, m11);
+                mstep<1>(k, us[q0 + i + 1], m00, m01, m10, m11);
+            }
+        } else {
+            for (int i = 0; i < len; ++i) {
+                if (i & 1)
+                    mstep<1>(k, us[q0 + i], m00, m01, m10, m11);
+                else
+                    mstep<0>(k, us[q0 + i], m00, m01, m10, m11);
+            }
+        }
+        renorm_mat(m00, m01, m10, m11, eT);
+    }
+}
+
 // (256, 4): up to 4 workgroups of 4 waves per CU, enough registers for the
 // lead self-energies without spills (4, 6 or 8 waves per SIMD time the same).
 template <int K>
@@ -283,7 +466,7 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
     // the first segment's wavefront also computes its chains' lead self-energies
     // in closed form (only the scan and the sweeps, later launches, read them)
     if (!lp.dec_use && ic.s == 0 && ic.valid) lead_sigma<false>(lp, lp.chains + ic.chain, ch, tube);
-    const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
+    const double2 E = make_double2(ch.Er, ch.Ei);
     const int lo = min(tube.N, ic.s * tube.L);
     const int hi = min(tube.N, lo + tube.L);
     double* us = us_all[wv];
@@ -291,23 +474,25 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
     double2 m00 = make_double2(1.0, 0.0), m01 = make_double2(0.0, 0.0);
     double2 m10 = make_double2(0.0, 0.0), m11 = make_double2(1.0, 0.0);
     int eT = 0;
-    for (int q0 = 0; q0 < hi - lo; q0 += K) {
-        const int len = min(K, hi - lo - q0);
-        if (len == K) {
-#pragma unroll
-            for (int i = 0; i < K; i += 2) {
-                mat_step<0>(k, us[q0 + i], m00, m01, m10, m11);
-                mat_step<1>(k, us[q0 + i + 1], m00, m01, m10, m11);
-            }
-        } else {
-            for (int i = 0; i < len; ++i) {
-                if (i & 1)
-                    mat_step<1>(k, us[q0 + i], m00, m01, m10, m11);
-                else
-                    mat_step<0>(k, us[q0 + i], m00, m01, m10, m11);
-            }
-        }
+    if (hi > lo && __all(scaled_ok(E, ch.t_even, tube.t_odd))) {
+        // scaled basis: column 1 starts at w_{lo-1} = u_{lo-1} ai_{lo-1};
+        // the rows return to u with s_L = C ai_{hi-1} and s_{L-1} = C
+        const SCoefs ks = make_scoefs(E, ch.t_even, tube.t_odd);
+        m11 = make_double2(((lo - 1) & 1) ? ks.ai[1] : ks.ai[0], 0.0);
+        transfer_loop<K>(ks, us, hi - lo, m00, m01, m10, m11, eT);
+        double mC;
+        int eC;
+        scale_prod(ks, lo & 1, hi - lo - 1, mC, eC);
+        const double r0 = mC * (((hi - 1) & 1) ? ks.ai[1] : ks.ai[0]);
+        m00 = cscale(r0, m00);
+        m01 = cscale(r0, m01);
+        m10 = cscale(mC, m10);
+        m11 = cscale(mC, m11);
+        eT += eC;
         renorm_mat(m00, m01, m10, m11, eT);
+    } else {
+        const Coefs k = make_coefs(E, ch.t_even, tube.t_odd);
+        transfer_loop<K>(k, us, hi - lo, m00, m01, m10, m11, eT);
     }
     double* T = lp.Tm + (long long)item * 9 * 64 + lane;
     T[0 * 64] = m00.x;
@@ -789,6 +974,73 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
     if (KIND == KIND_DOS) lp.dos_part[(long long)item * 64 + lane] = dos_acc;
 }
 
+// Passes 2 and 3 of the equilibrium sweep over a full segment of K NS sites
+// staged in us: (x, y) is the right state, (ux, uy) the left state, C0 the
+// complex weight that turns u into ut (coefficient / D and the exponents).
+template <int K, int NS, class KC>
+__device__ __forceinline__ void eq_full_body(const KC& k, const double* us, double2 ux, double2 uy, double2 x,
+                                             double2 y, int e, double2 cD, int eoff, double* red, int lane,
+                                             double* prow)
+{
+    // pass 2: v checkpoints at the right end of every sub-block
+    double2 cx[NS], cy[NS];
+    int ce[NS];
+    cx[NS - 1] = x;
+    cy[NS - 1] = y;
+    ce[NS - 1] = e;
+#pragma unroll
+    for (int j = NS - 1; j >= 1; --j) {
+#pragma unroll
+        for (int i = K - 1; i >= 1; i -= 2) {
+            bstep<1>(k, us[j * K + i], x, y);
+            bstep<0>(k, us[j * K + i - 1], x, y);
+        }
+        renorm(x, y, e);
+        cx[j - 1] = x;
+        cy[j - 1] = y;
+        ce[j - 1] = e;
+    }
+
+    // pass 3
+    {
+        const double sc = __builtin_amdgcn_ldexp(1.0, eoff + ce[0]);
+        const double2 C0 = cscale(sc, cD);
+        ux = cmul(C0, ux);
+        uy = cmul(C0, uy);
+    }
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double2 vr[K];
+        {
+            double2 xx = cx[j], yy = cy[j];
+            vr[K - 1] = xx;
+#pragma unroll
+            for (int i = K - 1; i >= 1; --i) {
+                if (i & 1)
+                    bstep<1>(k, us[j * K + i], xx, yy);
+                else
+                    bstep<0>(k, us[j * K + i], xx, yy);
+                vr[i - 1] = xx;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            red[i * 65 + lane] = fma(ux.x, vr[i].x, -ux.y * vr[i].y);
+            if (i & 1)
+                fstep<1>(k, us[j * K + i], ux, uy);
+            else
+                fstep<0>(k, us[j * K + i], ux, uy);
+        }
+        const double r = lane_sum_transpose<K>(red, lane);
+        if (lane < K) prow[j * K + lane] = r;
+        if (j + 1 < NS) {
+            const int d = ce[j + 1] - ce[j];
+            scale2(ux, d);
+            scale2(uy, d);
+        }
+    }
+}
+
 // Equilibrium sweep of a full segment of NS sub-blocks (the common case),
 // with the chain coefficient folded into the left state: with
 //   ut = (coef / D) 2^(e_u + e_v - e_D) u
@@ -796,6 +1048,10 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 // Re(ut_n v_n), two instructions.  Between sub-blocks ut only changes by the
 // exact power of two of the next v checkpoint, so u needs no renormalisation:
 // |ut_n v_n| = |coef G_nn| stays bounded.  Checkpoint indices are static.
+// Waves whose chains all have t / |Im E| <= 2^24 run the scaled recurrence:
+// the entry states become w_lo = u_lo, w_{lo-1} = ai_{lo-1} u_{lo-1},
+// w'_{hi-1} = v_{hi-1}, w'_hi = ai_{hi-1} v_hi, and the segment constant
+// C = ai_lo ... ai_{hi-2} joins the weight (u_n v_n = C w_n w'_n).
 template <int K, int NS>
 __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLaunch lp)
 {
@@ -811,16 +1067,16 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     ItemCtx ic = item_ctx(lp, item, lane, tube);
     if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
-    const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
+    const double2 E = make_double2(ch.Er, ch.Ei);
     const int N = tube.N;
-    const int lo = ic.s * tube.L;  // the segment is [lo, lo + K NS)
+    const int lo = ic.s * tube.L;  // the segment is [lo, lo + K NS), lo even
     double* us = us_all[wv];
     stage_segment(us, tube.U, lo, lo + K * NS, lane);
     const double* S = lp.St + (long long)item * 10 * 64 + lane;
     double2 ux = make_double2(S[0], S[64]), uy = make_double2(S[128], S[192]);
     const int ue = (int)S[256];
     double2 x = make_double2(S[320], S[384]), y = make_double2(S[448], S[512]);
-    int e = (int)S[576];
+    const int e = (int)S[576];
     const double* Dp = lp.Dd + (long long)ic.chain * 3;
     const double2 iD = crecip(make_double2(Dp[0], Dp[1]));
     const int eN = (int)Dp[2];
@@ -828,62 +1084,17 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     const double2 cD = cmul(make_double2(ch.cr * vmask, ch.ci * vmask), iD);
     double* prow = lp.part + tube.part_off + (long long)ic.g * N + lo;
 
-    // pass 2: v checkpoints at the right end of every sub-block
-    double2 cx[NS], cy[NS];
-    int ce[NS];
-    cx[NS - 1] = x;
-    cy[NS - 1] = y;
-    ce[NS - 1] = e;
-#pragma unroll
-    for (int j = NS - 1; j >= 1; --j) {
-#pragma unroll
-        for (int i = K - 1; i >= 1; i -= 2) {
-            bwd_step<1>(k, us[j * K + i], x, y);
-            bwd_step<0>(k, us[j * K + i - 1], x, y);
-        }
-        renorm(x, y, e);
-        cx[j - 1] = x;
-        cy[j - 1] = y;
-        ce[j - 1] = e;
-    }
-
-    // pass 3
-    {
-        const double sc = __builtin_amdgcn_ldexp(1.0, ue + ce[0] - eN);
-        const double2 C0 = cscale(sc, cD);
-        ux = cmul(C0, ux);
-        uy = cmul(C0, uy);
-    }
-#pragma unroll
-    for (int j = 0; j < NS; ++j) {
-        double2 vr[K];
-        {
-            double2 xx = cx[j], yy = cy[j];
-            vr[K - 1] = xx;
-#pragma unroll
-            for (int i = K - 1; i >= 1; --i) {
-                if (i & 1)
-                    bwd_step<1>(k, us[j * K + i], xx, yy);
-                else
-                    bwd_step<0>(k, us[j * K + i], xx, yy);
-                vr[i - 1] = xx;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < K; ++i) {
-            red[i * 65 + lane] = fma(ux.x, vr[i].x, -ux.y * vr[i].y);
-            if (i & 1)
-                fwd_step<1>(k, us[j * K + i], ux, uy);
-            else
-                fwd_step<0>(k, us[j * K + i], ux, uy);
-        }
-        const double r = lane_sum_transpose<K>(red, lane);
-        if (lane < K) prow[j * K + lane] = r;
-        if (j + 1 < NS) {
-            const int d = ce[j + 1] - ce[j];
-            scale2(ux, d);
-            scale2(uy, d);
-        }
+    if (__all(scaled_ok(E, ch.t_even, tube.t_odd))) {
+        const SCoefs ks = make_scoefs(E, ch.t_even, tube.t_odd);
+        uy = cscale(ks.ai[1], uy);  // site lo - 1 is odd
+        y = cscale(ks.ai[1], y);    // site hi - 1 is odd
+        double mC;
+        int eC;
+        scale_prod(ks, 0, K * NS - 1, mC, eC);
+        eq_full_body<K, NS>(ks, us, ux, uy, x, y, e, cscale(mC, cD), ue - eN + eC, red, lane, prow);
+    } else {
+        const Coefs k = make_coefs(E, ch.t_even, tube.t_odd);
+        eq_full_body<K, NS>(k, us, ux, uy, x, y, e, cD, ue - eN, red, lane, prow);
     }
 }
 
