@@ -449,6 +449,26 @@ __device__ __forceinline__ void transfer_loop(const KC& k, const double* us, int
     }
 }
 
+// Two independent products over the halves [0, h) and [h, 2h) of the staged
+// segment (h a multiple of K, the first site even), interleaved site by site.
+template <int K, class KC>
+__device__ __forceinline__ void transfer_loop2(const KC& k, const double* us, int h, double2& m00, double2& m01,
+                                               double2& m10, double2& m11, int& eA, double2& b00, double2& b01,
+                                               double2& b10, double2& b11, int& eB)
+{
+    for (int q0 = 0; q0 < h; q0 += K) {
+#pragma unroll
+        for (int i = 0; i < K; i += 2) {
+            mstep<0>(k, us[q0 + i], m00, m01, m10, m11);
+            mstep<0>(k, us[h + q0 + i], b00, b01, b10, b11);
+            mstep<1>(k, us[q0 + i + 1], m00, m01, m10, m11);
+            mstep<1>(k, us[h + q0 + i + 1], b00, b01, b10, b11);
+        }
+        renorm_mat(m00, m01, m10, m11, eA);
+        renorm_mat(b00, b01, b10, b11, eB);
+    }
+}
+
 // (256, 4): up to 4 workgroups of 4 waves per CU, enough registers for the
 // lead self-energies without spills (4, 6 or 8 waves per SIMD time the same).
 template <int K>
@@ -478,8 +498,27 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
         // scaled basis: column 1 starts at w_{lo-1} = u_{lo-1} ai_{lo-1};
         // the rows return to u with s_L = C ai_{hi-1} and s_{L-1} = C
         const SCoefs ks = make_scoefs(E, ch.t_even, tube.t_odd);
-        m11 = make_double2(((lo - 1) & 1) ? ks.ai[1] : ks.ai[0], 0.0);
-        transfer_loop<K>(ks, us, hi - lo, m00, m01, m10, m11, eT);
+        const double a0 = ((lo - 1) & 1) ? ks.ai[1] : ks.ai[0];
+        if (hi - lo == tube.L && tube.L % (2 * K) == 0) {
+            // The scaled coefficients do not depend on where the scales
+            // start, so the two halves of the segment are independent
+            // products (twice the independent FMA chains per lane), joined
+            // by one 2x2 product; column 1 then takes ai_{lo-1}.
+            double2 b00 = make_double2(1.0, 0.0), b01 = make_double2(0.0, 0.0);
+            double2 b10 = make_double2(0.0, 0.0), b11 = make_double2(1.0, 0.0);
+            int eB = 0;
+            transfer_loop2<K>(ks, us, tube.L / 2, m00, m01, m10, m11, eT, b00, b01, b10, b11, eB);
+            double2 r00, r01, r10, r11;
+            mat_mul(b00, b01, b10, b11, m00, m01, m10, m11, r00, r01, r10, r11);
+            m00 = r00;
+            m01 = cscale(a0, r01);
+            m10 = r10;
+            m11 = cscale(a0, r11);
+            eT += eB;
+        } else {
+            m11 = make_double2(a0, 0.0);
+            transfer_loop<K>(ks, us, hi - lo, m00, m01, m10, m11, eT);
+        }
         double mC;
         int eC;
         scale_prod(ks, lo & 1, hi - lo - 1, mC, eC);
