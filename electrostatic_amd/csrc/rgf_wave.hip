@@ -451,8 +451,8 @@ __device__ __forceinline__ void transfer_loop(const KC& k, const double* us, int
 
 // Two independent products over the halves [0, h) and [h, 2h) of the staged
 // segment (h a multiple of K, the first site even), interleaved site by site.
-template <int K, class KC>
-__device__ __forceinline__ void transfer_loop2(const KC& k, const double* us, int h, double2& m00, double2& m01,
+template <int K, class KC, class UP>
+__device__ __forceinline__ void transfer_loop2(const KC& k, UP us, int h, double2& m00, double2& m01,
                                                double2& m10, double2& m11, int& eA, double2& b00, double2& b01,
                                                double2& b10, double2& b11, int& eB)
 {
@@ -507,7 +507,13 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
             double2 b00 = make_double2(1.0, 0.0), b01 = make_double2(0.0, 0.0);
             double2 b10 = make_double2(0.0, 0.0), b11 = make_double2(1.0, 0.0);
             int eB = 0;
-            transfer_loop2<K>(ks, us, tube.L / 2, m00, m01, m10, m11, eT, b00, b01, b10, b11, eB);
+            // U is wave-uniform and read-only here: through the constant
+            // address space it arrives by scalar loads into SGPRs (U from the
+            // LDS copy took 64 VGPRs for the two halves and spilled 24 B per
+            // lane to scratch)
+            typedef const __attribute__((address_space(4))) double cdbl;
+            const cdbl* ug = (const cdbl*)(uintptr_t)(tube.U + lo);
+            transfer_loop2<K>(ks, ug, tube.L / 2, m00, m01, m10, m11, eT, b00, b01, b10, b11, eB);
             double2 r00, r01, r10, r11;
             mat_mul(b00, b01, b10, b11, m00, m01, m10, m11, r00, r01, r10, r11);
             m00 = r00;
