@@ -621,12 +621,14 @@ __device__ __forceinline__ void mat2_apply_inv(const Mat2& m, double idet, doubl
 }
 
 // CACHE > 0: every wave's range has at most CACHE segments (P <= CACHE *
-// SCAN_WAVES for every tube of the launch), and the transfer matrices stay in
-// registers from phase A to phase C instead of being read again.
-template <int CACHE>
-__global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int ngroups)
+// W for every tube of the launch), and the transfer matrices stay in
+// registers from phase A to phase C instead of being read again.  W: waves
+// per chain group (16 for long chains, whose sequential depth is then about
+// 2 P / 16 + 16 matrix steps instead of 2 P / 8 + 8).
+template <int CACHE, int W = SCAN_WAVES>
+__global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
 {
-    __shared__ double xm[SCAN_WAVES][9][64];
+    __shared__ double xm[W][9][64];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int gi = blockIdx.x;
@@ -646,7 +648,7 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     const double2 SR = make_double2(ch.SRr, ch.SRi);
     const long long item0 = tube.item0 + (long long)g * P;
     // segment range of each wave, and 1/det of a range [lo, hi) of sites
-    const int per = (P + SCAN_WAVES - 1) / SCAN_WAVES;
+    const int per = (P + W - 1) / W;
     auto range_lo = [&](int k) { return min(P, k * per); };
     auto inv_det = [&](int slo, int shi) {
         // sites [slo*L, min(N, shi*L)); t_{lo-1} is an odd link (lo even)
@@ -714,7 +716,7 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     double2 vx = cscale(1.0 / tau, SR);  // v_N
     double2 vy = make_double2(1.0, 0.0);  // v_{N-1}
     int ve = 0;
-    for (int k = SCAN_WAVES - 1; k > w; --k) {
+    for (int k = W - 1; k > w; --k) {
         const int k0 = range_lo(k), k1 = range_lo(k + 1);
         if (k0 < k1) mat2_apply_inv(get(k), inv_det(k0, k1), vx, vy, ve);
     }
@@ -744,7 +746,7 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
                 if (c + q < s1) u_step(c + q, Tc[q]);
         }
     }
-    if (w == SCAN_WAVES - 1 && valid) {
+    if (w == W - 1 && valid) {
         // this wave's u now sits at the end of the chain: (u_N, u_{N-1})
         const double2 Dm = csub(cscale(tau, ux), cmul(SR, uy));  // D = Dm 2^ue
         const int eN = ue;

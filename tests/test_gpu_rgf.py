@@ -218,3 +218,44 @@ def test_bad_arguments_raise():
     rc = lib.negf_rgf_diag_accumulate(p, None, 0, None, None, None, None, None)
     assert rc == 1
     assert b"invalid" in lib.negf_last_error()
+
+
+@pytest.mark.parametrize("N", [1000, 2048])
+def test_scaled_and_unscaled_recurrences_agree(N):
+    """The transfer and equilibrium sweep kernels divide the recurrence by
+    Im E when every chain of a 64-chain wave has t / |Im E| <= 2^24, and keep
+    the plain recurrence otherwise.  Wave 0 (eta = 1e-5) and wave 2
+    (eta = 2e-7, just above the bound 2.5 / 2^24 = 1.5e-7) take the scaled
+    form; wave 1 repeats wave 0's energies but one lane has eta = 1e-9, so the
+    whole wave falls back to the plain form (that lane's weight is 0 in both
+    waves); wave 3 is a contour arc.  Each wave alone and all four together
+    match the oracle to 1e-10, and the scaled and plain runs of the same
+    energies agree to 1e-11 (measured 1.3e-12: rounding near the band edges
+    is amplified by about 1/eta, and the plain recurrence alone moves the
+    charge by ~1e-12 between segment lengths)."""
+    rng = np.random.default_rng(7 + N)
+    U = smooth_potential(N, seed=N + 3)
+    Uc = (U[0], U[-1])
+    re = np.sort(rng.uniform(-3, 3, 64))
+    w0 = re + 1e-5j
+    w1 = w0.copy()
+    w1[17] = 1.5 + 1e-9j  # in the band of both leads
+    w2 = np.sort(rng.uniform(-3, 3, 64)) + 2e-7j
+    th = np.sort(rng.uniform(0.05, np.pi - 0.05, 64))
+    w3 = -4.5 + 5.5 * np.exp(1j * th)
+    c = rng.normal(size=64) + 1j * rng.normal(size=64)
+    c[17] = 0.0
+    waves = [w0, w1, w2, w3]
+    accs = []
+    for w in waves:
+        ref, st = O.accumulate_diagG(U, BETA17, GAMMA, Uc, BETA17, GAMMA, w, c)
+        assert st == 0
+        acc, _ = run_diag(U, BETA17, GAMMA, Uc, w, c, want_G=False)
+        assert rel_err(acc, ref) < 1e-10
+        accs.append(acc)
+    assert rel_err(accs[1], accs[0]) < 1e-11
+    E = np.concatenate(waves)
+    coef = np.tile(c, 4)
+    ref, st = O.accumulate_diagG(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E, coef)
+    acc, _ = run_diag(U, BETA17, GAMMA, Uc, E, coef, want_G=False)
+    assert rel_err(acc, ref) < 1e-10
