@@ -1283,6 +1283,9 @@ __global__ __launch_bounds__(256) void reduce_rows1_kernel(const double* __restr
     double acc = 0.0;
     if (n < N) {
         const double* p = src + src_off[t] + n;
+        // unrolled so that the loads are in flight together; the additions
+        // keep their order
+#pragma unroll 8
         for (int r = k; r < R; r += 16) acc += p[(long long)r * N];
     }
     red[k][sl] = acc;
