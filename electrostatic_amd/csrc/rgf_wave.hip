@@ -1145,6 +1145,86 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     }
 }
 
+// Scaled form of the non-equilibrium full-segment sweep (every chain of the
+// wave has t / |Im E| <= 2^24).  With u_n = s_n w_n and v_n = s'_n w'_n
+// (s_lo = 1, s'_{hi-1} = 1, s_{n+1} = s_n ai_n, s'_{n+1} = s'_n / ai_n) the
+// contribution of site n is
+//   Q_n |w_n|^2 + P_n |w'_n|^2,
+//   Q_n = sgnR rR^2 2^(2(e_w - e_D)) s_n^2,  P_n = sgnL rL^2 2^(2(e_v - e_D)) s'_n^2,
+// where e_w is the exponent of the renormalised forward state and e_v that of
+// the sub-block's checkpoint.  Q and P are carried site by site (times ai_n^2
+// and 1 / ai_n^2) and by exact powers of two where w is renormalised or a new
+// checkpoint starts; P starts from s'_lo = C.
+template <int K, int NS>
+__device__ __forceinline__ void neq_full_scaled(const SCoefs& k, const double* us, double2 ux, double2 uy, int ue,
+                                                double2 x, double2 y, int e, int eN, double sgnL, double sgnR,
+                                                double rL, double rR, double* red, int lane, double* prow)
+{
+    uy = cscale(k.ai[1], uy);  // site lo - 1 is odd
+    y = cscale(k.ai[1], y);    // site hi - 1 is odd
+    double mC;
+    int eC;
+    scale_prod(k, 0, K * NS - 1, mC, eC);
+    double2 cx[NS], cy[NS];
+    int ce[NS];
+    cx[NS - 1] = x;
+    cy[NS - 1] = y;
+    ce[NS - 1] = e;
+#pragma unroll
+    for (int j = NS - 1; j >= 1; --j) {
+#pragma unroll
+        for (int i = K - 1; i >= 1; i -= 2) {
+            bwd_s<1>(k, us[j * K + i], x, y);
+            bwd_s<0>(k, us[j * K + i - 1], x, y);
+        }
+        renorm(x, y, e);
+        cx[j - 1] = x;
+        cy[j - 1] = y;
+        ce[j - 1] = e;
+    }
+    int ew = ue;
+    renorm(ux, uy, ew);
+    double Q = sgnR * rR * rR * __builtin_amdgcn_ldexp(1.0, 2 * (ew - eN));
+    double P = sgnL * rL * rL * mC * mC * __builtin_amdgcn_ldexp(1.0, 2 * (ce[0] - eN + eC));
+    const double a2[2] = {k.ai[0] * k.ai[0], k.ai[1] * k.ai[1]};
+    const double ia2[2] = {1.0 / a2[0], 1.0 / a2[1]};
+#pragma unroll
+    for (int j = 0; j < NS; ++j) {
+        double2 vr[K];
+        {
+            double2 xx = cx[j], yy = cy[j];
+            vr[K - 1] = xx;
+#pragma unroll
+            for (int i = K - 1; i >= 1; --i) {
+                if (i & 1)
+                    bwd_s<1>(k, us[j * K + i], xx, yy);
+                else
+                    bwd_s<0>(k, us[j * K + i], xx, yy);
+                vr[i - 1] = xx;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+            red[i * 65 + lane] = fma(P, cabs2(vr[i]), Q * cabs2(ux));
+            if (i & 1)
+                fwd_s<1>(k, us[j * K + i], ux, uy);
+            else
+                fwd_s<0>(k, us[j * K + i], ux, uy);
+            Q *= a2[i & 1];
+            P *= ia2[i & 1];
+        }
+        const double r = lane_sum_transpose<K>(red, lane);
+        if (lane < K) prow[j * K + lane] = r;
+        if (j + 1 < NS) {
+            P = __builtin_amdgcn_ldexp(P, 2 * (ce[j + 1] - ce[j]));
+            int kk = exp_of(amax4(ux, uy));
+            scale2(ux, -kk);
+            scale2(uy, -kk);
+            Q = __builtin_amdgcn_ldexp(Q, 2 * kk);
+        }
+    }
+}
+
 // Non-equilibrium sweep of a full segment (no integrand samples): the
 // contribution of site n is kL Gamma_L |G_n0|^2 + kR Gamma_R |G_n,N-1|^2 with
 // G_n0 = v_n / D and G_n,N-1 = u_n / D.  The weights are folded into the
@@ -1167,7 +1247,6 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_neq_full_kernel(WL
     ItemCtx ic = item_ctx(lp, item, lane, tube);
     if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
-    const Coefs k = make_coefs(make_double2(ch.Er, ch.Ei), ch.t_even, tube.t_odd);
     const int N = tube.N;
     const int lo = ic.s * tube.L;
     double* us = us_all[wv];
@@ -1186,6 +1265,13 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_neq_full_kernel(WL
     const double sgnL = wL < 0.0 ? -1.0 : 1.0, sgnR = wR < 0.0 ? -1.0 : 1.0;
     const double rL = sqrt(fabs(wL)), rR = sqrt(fabs(wR));
     double* prow = lp.part + tube.part_off + (long long)ic.g * N + lo;
+    const double2 E = make_double2(ch.Er, ch.Ei);
+    if (__all(scaled_ok(E, ch.t_even, tube.t_odd))) {
+        neq_full_scaled<K, NS>(make_scoefs(E, ch.t_even, tube.t_odd), us, ux, uy, ue, x, y, e, eN, sgnL, sgnR,
+                               rL, rR, red, lane, prow);
+        return;
+    }
+    const Coefs k = make_coefs(E, ch.t_even, tube.t_odd);
 
     double2 cx[NS], cy[NS];
     int ce[NS];
