@@ -18,7 +18,9 @@
 //   G_nn = u_n v_n / D,  G_{n,0} = v_n / D,  G_{n,N-1} = u_n / D.
 // u is the left-connected (forward-stable) continuant, v the right-connected
 // (backward-stable) one; their ratios are the RGF Y_n and X_n.  No per-site
-// division remains.
+// division remains.  Where |Im E| allows, the full-segment kernels run the
+// same recurrences divided by Im E site by site (four FMAs per complex step
+// instead of six; see "scaled recurrence" below).
 //
 // Mapping: a lane is one chain (one energy of one mode of one tube); the 64
 // lanes of a wavefront are 64 chains of the same tube, so the site index and
@@ -1022,8 +1024,10 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 }
 
 // Passes 2 and 3 of the equilibrium sweep over a full segment of K NS sites
-// staged in us: (x, y) is the right state, (ux, uy) the left state, C0 the
-// complex weight that turns u into ut (coefficient / D and the exponents).
+// staged in us: (x, y) is the right state (exponent e), (ux, uy) the left
+// state; ut = cD 2^(eoff + e_v) u with e_v the exponent of the first v
+// checkpoint (cD = coefficient / D, eoff = e_u - e_D and, for the scaled
+// recurrence, the exponent of the segment constant C).
 template <int K, int NS, class KC>
 __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, double2 ux, double2 uy, double2 x,
                                              double2 y, int e, double2 cD, int eoff, double* red, int lane,
