@@ -676,9 +676,16 @@ __global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
 #pragma unroll
         for (int q = 0; q < NT; ++q)
             if (s0 + q < s1) Tk[q] = load_T(lp, item0 + s0 + q, lane);
+        if (NT == 4 && s1 - s0 == 4) {
+            // product tree: depth two instead of a chain of four
+            const Mat2 p01 = mat2_mul(Tk[1], Tk[0]);
+            const Mat2 p23 = mat2_mul(Tk[3 % NT], Tk[2 % NT]);
+            M = mat2_mul(p23, p01);
+        } else {
 #pragma unroll
-        for (int q = 0; q < NT; ++q)
-            if (s0 + q < s1) M = mat2_mul(Tk[q], M);
+            for (int q = 0; q < NT; ++q)
+                if (s0 + q < s1) M = mat2_mul(Tk[q], M);
+        }
     } else {
         for (int c = s0; c < s1; c += 4) {
             // issue the loads of up to four segments before the products
@@ -723,7 +730,9 @@ __global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
         if (k0 < k1) mat2_apply_inv(get(k), inv_det(k0, k1), vx, vy, ve);
     }
 
-    // phase C: per-segment states through this wave's range
+    // phase C: per-segment states through this wave's range (the u and v
+    // walks are independent; with the matrices in registers they advance in
+    // one loop)
     auto u_step = [&](int s, const Mat2& Ts) {
         double* S = lp.St + (item0 + s) * 10 * 64 + lane;
         S[0 * 64] = ux.x;
@@ -733,10 +742,23 @@ __global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
         S[4 * 64] = (double)ue;
         mat2_apply(Ts, ux, uy, ue);
     };
+    auto v_step = [&](int s, const Mat2& Ts) {
+        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
+        S[5 * 64] = vy.x;  // (v_{hi-1}, v_hi)
+        S[6 * 64] = vy.y;
+        S[7 * 64] = vx.x;
+        S[8 * 64] = vx.y;
+        S[9 * 64] = (double)ve;
+        const int lo = min(N, s * L);
+        const int hi = min(N, lo + L);
+        if (lo < hi) mat2_apply_inv(Ts, inv_det(s, s + 1), vx, vy, ve);
+    };
     if (cached) {
 #pragma unroll
-        for (int q = 0; q < NT; ++q)
+        for (int q = 0; q < NT; ++q) {
             if (s0 + q < s1) u_step(s0 + q, Tk[q]);
+            if (s0 + NT - 1 - q < s1) v_step(s0 + NT - 1 - q, Tk[NT - 1 - q]);
+        }
     } else {
         for (int c = s0; c < s1; c += 4) {
             Mat2 Tc[4];
@@ -746,6 +768,15 @@ __global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
 #pragma unroll
             for (int q = 0; q < 4; ++q)
                 if (c + q < s1) u_step(c + q, Tc[q]);
+        }
+        for (int c = s1 - 1; c >= s0; c -= 4) {
+            Mat2 Tc[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c - q >= s0) Tc[q] = load_T(lp, item0 + c - q, lane);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (c - q >= s0) v_step(c - q, Tc[q]);
         }
     }
     if (w == W - 1 && valid) {
@@ -762,32 +793,6 @@ __global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
             double gN = __builtin_amdgcn_ldexp(1.0, -eN);
             double GamL = -2.0 * SL.y, GamR = -2.0 * SR.y;
             lp.chain_out[(long long)ch.out_slot * 2 + 1] = GamL * GamR * iD2 * gN * gN;
-        }
-    }
-    auto v_step = [&](int s, const Mat2& Ts) {
-        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
-        S[5 * 64] = vy.x;  // (v_{hi-1}, v_hi)
-        S[6 * 64] = vy.y;
-        S[7 * 64] = vx.x;
-        S[8 * 64] = vx.y;
-        S[9 * 64] = (double)ve;
-        const int lo = min(N, s * L);
-        const int hi = min(N, lo + L);
-        if (lo < hi) mat2_apply_inv(Ts, inv_det(s, s + 1), vx, vy, ve);
-    };
-    if (cached) {
-#pragma unroll
-        for (int q = NT - 1; q >= 0; --q)
-            if (s0 + q < s1) v_step(s0 + q, Tk[q]);
-    } else {
-        for (int c = s1 - 1; c >= s0; c -= 4) {
-            Mat2 Tc[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (c - q >= s0) Tc[q] = load_T(lp, item0 + c - q, lane);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (c - q >= s0) v_step(c - q, Tc[q]);
         }
     }
 }
