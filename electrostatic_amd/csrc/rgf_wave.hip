@@ -1163,7 +1163,11 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
 // where e_w is the exponent of the renormalised forward state and e_v that of
 // the sub-block's checkpoint.  Q and P are carried site by site (times ai_n^2
 // and 1 / ai_n^2) and by exact powers of two where w is renormalised or a new
-// checkpoint starts; P starts from s'_lo = C.
+// checkpoint starts; P starts from s'_lo = C.  Range: with t / |Im E| <= 2^24
+// a state grows by at most about 2^27 per site, so |w|^2 within a 16-site
+// sub-block stays below 2^864 and Q, P (about contribution / |w|^2) stay
+// normal for every contribution above about 2^-150; smaller ones (far below
+// the charge's rounding) may flush to zero, as they do in the plain sweep.
 template <int K, int NS>
 __device__ __forceinline__ void neq_full_scaled(const SCoefs& k, const double* us, double2 ux, double2 uy, int ue,
                                                 double2 x, double2 y, int e, int eN, double sgnL, double sgnR,
