@@ -82,6 +82,9 @@ class Batch {
     int build()
     {
         drop_graph();
+        // every rebuild may move the plans' buffers: graphs captured by
+        // callers (the batched SCF) compare this counter before a replay
+        ++generation;
         const int n = (int)tubes.size();
         std::vector<ChainDesc> eq, neq;
         std::vector<RgfTube> teq, tneq;
@@ -247,12 +250,23 @@ class Batch {
         return NEGF_OK;
     }
 
-    int check_status()
+    // Read the plans' device status words into the sticky root flag
+    // (synchronises the batch stream).
+    int poll_status()
     {
         int s1 = 0, s2 = 0, rc;
         if ((rc = eq_plan.check_status(stream, &s1))) return rc;
         if (n_neq > 0 && (rc = neq_plan.check_status(stream, &s2))) return rc;
-        if ((s1 | s2) & 1) {
+        root_flag |= (s1 | s2) & 1;
+        return NEGF_OK;
+    }
+    int check_status()
+    {
+        int rc = poll_status();
+        if (rc) return rc;
+        const int f = root_flag;
+        root_flag = 0;
+        if (f) {
             set_error("surface Green's function: no root with Im<0 at some energy (g = 0 used)");
             return NEGF_EROOT;
         }
@@ -282,6 +296,8 @@ class Batch {
     std::vector<long long> versions;
     int seg_len = 0;
     int n_eq = 0, n_neq = 0;
+    long long generation = 0;  // incremented by every build()
+    int root_flag = 0;         // sticky: a surface GF without an Im<0 root since the last status call
     double energy_sites = 0.0;
     RgfPlan eq_plan, neq_plan;
     DevBuf eq_chains, neq_chains, d_rho0, d_eq, d_neq;
@@ -306,6 +322,8 @@ int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st, const int* tube_act
 {
     return b->b.enqueue_on(dQ, st, tube_active);
 }
+long long batch_generation(const negf_batch* b) { return b->b.generation; }
+int batch_poll_status(negf_batch* b) { return b->b.poll_status(); }
 }  // namespace negf
 
 extern "C" {

@@ -127,6 +127,12 @@ int choose_segments(int N, int L_req, int* P, int* L);
 // on a given stream (capturable).
 int batch_prepare(negf_batch* b, int iter, bool* rebuilt);
 int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st, const int* tube_active = nullptr);
+// Build counter of the batch's plans: a graph captured over batch_enqueue
+// is valid only while this value is unchanged.
+long long batch_generation(const negf_batch* b);
+// Fold the plans' device status words into the batch's sticky root flag
+// (reported and cleared by negf_batch_status); synchronises.
+int batch_poll_status(negf_batch* b);
 int max_sites();
 
 }  // namespace negf

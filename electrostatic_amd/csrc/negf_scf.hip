@@ -525,6 +525,7 @@ class BatchScf {
         T = T_;
         p = prm;
         N = tubes[0]->t.N;
+        this->tubes.assign(tubes, tubes + T_);
         for (int t = 0; t < T; ++t) {
             if (tubes[t]->t.N != N) {
                 set_error("batched SCF needs tubes of equal length (shared electrostatic kernel)");
@@ -606,96 +607,164 @@ class BatchScf {
         if (gexec_pc) (void)hipGraphExecDestroy(gexec_pc);
         gexec_pc = nullptr;
     }
+    // Capture fn() on st into *ge unless a graph exists that was captured
+    // over the batch's present plans (same build generation).
+    template <class F>
+    int capture(hipGraphExec_t* ge, long long* gen, F fn)
+    {
+        const long long g_now = batch_generation(batch);
+        if (*ge && *gen == g_now) return NEGF_OK;
+        if (*ge) (void)hipGraphExecDestroy(*ge);
+        *ge = nullptr;
+        NEGF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int erc = fn();
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(st, &g);
+        if (erc) {
+            if (g) (void)hipGraphDestroy(g);
+            return erc;
+        }
+        if (e != hipSuccess) {
+            set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+            return NEGF_EDEVICE;
+        }
+        const hipError_t e2 = hipGraphInstantiate(ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+            *ge = nullptr;
+            set_error("hipGraphInstantiate: %s", hipGetErrorString(e2));
+            return NEGF_EDEVICE;
+        }
+        *gen = g_now;
+        return NEGF_OK;
+    }
+
+    // Host part of iteration it: when a tube's adaptive correction falls due
+    // (flag_adaptive_integration_limits), its integrand is sampled at this
+    // iteration's potential, so the potential is written first, as in
+    // Solve_NEGF (NEGF_Common.cpp:1314-1337); then the batch's plans are
+    // brought up to date.
+    int prepare_iteration(int it)
+    {
+        int rc;
+        if (adaptive_due(it)) {
+            if ((rc = enqueue_potential())) return rc;
+            NEGF_HIP_TRY(hipStreamSynchronize(st));
+        }
+        bool rebuilt = false;
+        return batch_prepare(batch, it, &rebuilt);
+    }
+    bool adaptive_due(int it) const
+    {
+        for (const negf_tube* t : tubes)
+            if (t->t.p.flag_adaptive_integration_limits && it % t->t.p.integrand_correction_interval == 0) return true;
+        return false;
+    }
+
+    // After the iteration's synchronisation: fold the plans' status into the
+    // batch's sticky root flag, stop on a non-finite norm, retire converged
+    // tubes (their U and Q are kept).
+    struct LoopState {
+        std::vector<int> active, keep, done_it;
+        std::vector<double> fin;
+        int remaining = 0;
+        bool overflow = false;
+    };
+    int after_iteration(int it, LoopState& ls)
+    {
+        int rc;
+        if ((rc = batch_poll_status(batch))) return rc;
+        const double* o = h_out.as<double>();
+        bool any = false;
+        for (int t = 0; t < T; ++t) {
+            ls.keep[t] = 0;
+            if (!ls.active[t]) continue;
+            if (o[4 * t + 3] != 0.0) {
+                ls.overflow = true;
+                ls.active[t] = 0;
+                --ls.remaining;
+                continue;
+            }
+            if (!std::isfinite(o[4 * t])) {
+                set_error("batched SCF: non-finite Broyden norm for tube %d at iteration %d", t, it + 1);
+                return NEGF_ENAN;
+            }
+            if (o[4 * t] <= p.Broyden_max_norm) {
+                ls.keep[t] = 1;
+                ls.active[t] = 0;
+                ls.done_it[t] = it + 1;
+                ls.fin[t] = o[4 * t];
+                --ls.remaining;
+                any = true;
+            }
+        }
+        if (any) {
+            NEGF_HIP_TRY(hipMemcpyAsync(d_keep.p, ls.keep.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+            const long long NT = (long long)N * T;
+            hipLaunchKernelGGL(batch_keep_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
+                               d_keep.as<int>(), d_U.as<double>(), d_Q.as<double>(), d_Uo.as<double>(),
+                               d_Qo.as<double>());
+            NEGF_HIP_TRY(hipGetLastError());
+            NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, ls.active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        }
+        return NEGF_OK;
+    }
+    int finish_run(int it, const LoopState& ls, double* U_out, double* Q_out, int* iters, double* norms,
+                   int* total_iters)
+    {
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        if (U_out) NEGF_HIP_TRY(hipMemcpy(U_out, d_Uo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        if (Q_out) NEGF_HIP_TRY(hipMemcpy(Q_out, d_Qo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        if (iters) std::copy(ls.done_it.begin(), ls.done_it.end(), iters);
+        if (norms) std::copy(ls.fin.begin(), ls.fin.end(), norms);
+        if (total_iters) *total_iters = it;
+        if (ls.overflow) {
+            set_error("Broyden_Step has exceeded the Broyden_Threshold_MaxStep for some tube");
+            return NEGF_ENONCONV;
+        }
+        return NEGF_OK;
+    }
+    int start_run(const double* V, LoopState& ls)
+    {
+        NEGF_HIP_TRY(hipMemcpyAsync(d_V.p, V, sizeof(double) * T, hipMemcpyHostToDevice, st));
+        std::vector<int> steps(T, 1);
+        ls.active.assign(T, 1);
+        ls.keep.assign(T, 0);
+        ls.done_it.assign(T, -1);
+        ls.fin.assign(T, 0.0);
+        ls.remaining = T;
+        ls.overflow = false;
+        NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, ls.active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        NEGF_HIP_TRY(hipStreamSynchronize(st));
+        return NEGF_OK;
+    }
+
     // Run every tube to Broyden_Norm <= Broyden_max_norm (or its step
     // limit); per-tube iteration counts and final norms.
     int run(const double* V, double* U_out, double* Q_out, int* iters, double* norms, int* total_iters)
     {
         int rc;
-        NEGF_HIP_TRY(hipMemcpyAsync(d_V.p, V, sizeof(double) * T, hipMemcpyHostToDevice, st));
-        std::vector<int> steps(T, 1), active(T, 1), keep(T, 0), done_it(T, -1);
-        std::vector<double> fin(T, 0.0);
-        NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-        NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-        NEGF_HIP_TRY(hipStreamSynchronize(st));
-        int it = 0, remaining = T;
-        bool overflow = false;
-        while (remaining > 0) {
+        LoopState ls;
+        if ((rc = start_run(V, ls))) return rc;
+        int it = 0;
+        while (ls.remaining > 0) {
             if (it > M + 1) {
-                overflow = true;
+                ls.overflow = true;
                 break;
             }
-            bool rebuilt = false;
-            if ((rc = batch_prepare(batch, it, &rebuilt))) return rc;
+            if ((rc = prepare_iteration(it))) return rc;
             if (use_graph) {
-                if (!gexec || rebuilt) {
-                    drop_graph();
-                    NEGF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                    int erc = enqueue_iteration();
-                    hipGraph_t g = nullptr;
-                    hipError_t e = hipStreamEndCapture(st, &g);
-                    if (erc) {
-                        if (g) (void)hipGraphDestroy(g);
-                        return erc;
-                    }
-                    if (e != hipSuccess) {
-                        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
-                        return NEGF_EDEVICE;
-                    }
-                    e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    if (e != hipSuccess) {
-                        gexec = nullptr;
-                        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
-                        return NEGF_EDEVICE;
-                    }
-                }
+                if ((rc = capture(&gexec, &gen_graph, [&] { return enqueue_iteration(); }))) return rc;
                 NEGF_HIP_TRY(hipGraphLaunch(gexec, st));
-            } else {
-                if ((rc = enqueue_iteration())) return rc;
+            } else if ((rc = enqueue_iteration())) {
+                return rc;
             }
             NEGF_HIP_TRY(hipStreamSynchronize(st));
-            const double* o = h_out.as<double>();
-            bool any = false;
-            for (int t = 0; t < T; ++t) {
-                keep[t] = 0;
-                if (!active[t]) continue;
-                if (o[4 * t + 3] != 0.0) {
-                    overflow = true;
-                    active[t] = 0;
-                    --remaining;
-                    continue;
-                }
-                if (o[4 * t] <= p.Broyden_max_norm) {
-                    keep[t] = 1;
-                    active[t] = 0;
-                    done_it[t] = it + 1;
-                    fin[t] = o[4 * t];
-                    --remaining;
-                    any = true;
-                }
-            }
-            if (any) {
-                NEGF_HIP_TRY(hipMemcpyAsync(d_keep.p, keep.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-                const long long NT = (long long)N * T;
-                hipLaunchKernelGGL(batch_keep_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
-                                   d_keep.as<int>(), d_U.as<double>(), d_Q.as<double>(), d_Uo.as<double>(),
-                                   d_Qo.as<double>());
-                NEGF_HIP_TRY(hipGetLastError());
-                NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-            }
+            if ((rc = after_iteration(it, ls))) return rc;
             ++it;
         }
-        NEGF_HIP_TRY(hipStreamSynchronize(st));
-        if (U_out) NEGF_HIP_TRY(hipMemcpy(U_out, d_Uo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
-        if (Q_out) NEGF_HIP_TRY(hipMemcpy(Q_out, d_Qo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
-        if (iters) std::copy(done_it.begin(), done_it.end(), iters);
-        if (norms) std::copy(fin.begin(), fin.end(), norms);
-        if (total_iters) *total_iters = it;
-        if (overflow) {
-            set_error("Broyden_Step has exceeded the Broyden_Threshold_MaxStep for some tube");
-            return NEGF_ENONCONV;
-        }
-        return NEGF_OK;
+        return finish_run(it, ls, U_out, Q_out, iters, norms, total_iters);
     }
 
     // ---- surrogate-seeded, preconditioned iteration
@@ -731,7 +800,7 @@ class BatchScf {
         launch_dgemm(N, T, nz, d_K.as<double>(), base().n_in, state_stride, d_KN.as<double>(), st);
         const long long NT = (long long)N * T;
         hipLaunchKernelGGL(batch_potential_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
-                           has_ext ? d_Uext.as<double>() : nullptr, d_V.as<double>(), d_KN.as<double>(), nz,
+                           d_Uext.as<double>(), d_V.as<double>(), d_KN.as<double>(), nz,
                            d_ptrU.as<double*>(), d_ptrUc.as<double*>(), d_U.as<double>());
         NEGF_HIP_TRY(hipGetLastError());
         return NEGF_OK;
@@ -785,11 +854,8 @@ class BatchScf {
                double* norms, int* total_iters, double* cap)
     {
         int rc;
-        NEGF_HIP_TRY(hipMemcpyAsync(d_V.p, V, sizeof(double) * T, hipMemcpyHostToDevice, st));
-        std::vector<int> steps(T, 1), active(T, 1), keep(T, 0), done_it(T, -1);
-        std::vector<double> fin(T, 0.0);
-        NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-        NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+        LoopState ls;
+        if ((rc = start_run(V, ls))) return rc;
         if ((rc = enqueue_potential())) return rc;
         if ((rc = enqueue_capacitance())) return rc;
         const long long NT = (long long)N * T;
@@ -806,100 +872,43 @@ class BatchScf {
         }
         if (seed_iters > 0 && (rc = enqueue_capacitance())) return rc;
         NEGF_HIP_TRY(hipStreamSynchronize(st));
-        int it = 0, remaining = T;
-        bool overflow = false;
-        while (remaining > 0) {
+        int it = 0;
+        while (ls.remaining > 0) {
             if (it > M + 1) {
-                overflow = true;
+                ls.overflow = true;
                 break;
             }
             if (it > 0 && it == refresh_iter) {
                 // new capacitances at the present potential; fresh history, n_in kept
                 if ((rc = enqueue_capacitance())) return rc;
-                for (int t = 0; t < T; ++t) steps[t] = 1;
+                std::vector<int> steps(T, 1);
                 NEGF_HIP_TRY(hipMemcpyAsync(d_steps.p, steps.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+                NEGF_HIP_TRY(hipStreamSynchronize(st));
             }
-            bool rebuilt = false;
-            if ((rc = batch_prepare(batch, it, &rebuilt))) return rc;
+            if ((rc = prepare_iteration(it))) return rc;
             if (use_graph) {
-                if (!gexec_pc || rebuilt) {
-                    if (gexec_pc) (void)hipGraphExecDestroy(gexec_pc);
-                    gexec_pc = nullptr;
-                    NEGF_HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-                    const int erc = enqueue_iteration_pc();
-                    hipGraph_t g = nullptr;
-                    const hipError_t e = hipStreamEndCapture(st, &g);
-                    if (erc) {
-                        if (g) (void)hipGraphDestroy(g);
-                        return erc;
-                    }
-                    if (e != hipSuccess) {
-                        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
-                        return NEGF_EDEVICE;
-                    }
-                    const hipError_t e2 = hipGraphInstantiate(&gexec_pc, g, nullptr, nullptr, 0);
-                    (void)hipGraphDestroy(g);
-                    if (e2 != hipSuccess) {
-                        gexec_pc = nullptr;
-                        set_error("hipGraphInstantiate: %s", hipGetErrorString(e2));
-                        return NEGF_EDEVICE;
-                    }
-                }
+                if ((rc = capture(&gexec_pc, &gen_graph_pc, [&] { return enqueue_iteration_pc(); }))) return rc;
                 NEGF_HIP_TRY(hipGraphLaunch(gexec_pc, st));
             } else if ((rc = enqueue_iteration_pc())) {
                 return rc;
             }
             NEGF_HIP_TRY(hipStreamSynchronize(st));
-            const double* o = h_out.as<double>();
-            bool any = false;
-            for (int t = 0; t < T; ++t) {
-                keep[t] = 0;
-                if (!active[t]) continue;
-                if (o[4 * t + 3] != 0.0) {
-                    overflow = true;
-                    active[t] = 0;
-                    --remaining;
-                    continue;
-                }
-                if (o[4 * t] <= p.Broyden_max_norm) {
-                    keep[t] = 1;
-                    active[t] = 0;
-                    done_it[t] = it + 1;
-                    fin[t] = o[4 * t];
-                    --remaining;
-                    any = true;
-                }
-            }
-            if (any) {
-                NEGF_HIP_TRY(hipMemcpyAsync(d_keep.p, keep.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(batch_keep_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
-                                   d_keep.as<int>(), d_U.as<double>(), d_Q.as<double>(), d_Uo.as<double>(),
-                                   d_Qo.as<double>());
-                NEGF_HIP_TRY(hipGetLastError());
-                NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-            }
+            if ((rc = after_iteration(it, ls))) return rc;
             ++it;
         }
-        NEGF_HIP_TRY(hipStreamSynchronize(st));
-        if (U_out) NEGF_HIP_TRY(hipMemcpy(U_out, d_Uo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
-        if (Q_out) NEGF_HIP_TRY(hipMemcpy(Q_out, d_Qo.p, sizeof(double) * N * T, hipMemcpyDeviceToHost));
+        rc = finish_run(it, ls, U_out, Q_out, iters, norms, total_iters);
         if (cap) NEGF_HIP_TRY(hipMemcpy(cap, d_c.p, sizeof(double) * T, hipMemcpyDeviceToHost));
-        if (iters) std::copy(done_it.begin(), done_it.end(), iters);
-        if (norms) std::copy(fin.begin(), fin.end(), norms);
-        if (total_iters) *total_iters = it;
-        if (overflow) {
-            set_error("Broyden_Step has exceeded the Broyden_Threshold_MaxStep for some tube");
-            return NEGF_ENONCONV;
-        }
-        return NEGF_OK;
+        return rc;
     }
 
     Surrogate* sg = nullptr;
     bool has_pc = false, has_feat = false;
     DevBuf d_Z, d_Zt, d_lam, d_c, d_R, d_Rh, d_PF, d_ones, d_J, d_feat;
     hipGraphExec_t gexec_pc = nullptr;
+    long long gen_graph = -1, gen_graph_pc = -1;  // batch generation each graph was captured over
 
     negf_batch* batch = nullptr;
+    std::vector<negf_tube*> tubes;
     int T = 0, N = 0, M = 0;
     int nz = 1;  // K-split of the GEMMs (partials in d_KN)
     negf_scf_params p{};
@@ -1188,12 +1197,25 @@ int negf_batch_scf_set_external_potential(negf_batch_scf* s, const double* U_ext
         set_error("null argument");
         return NEGF_EARG;
     }
+    // The captured iteration reads d_Uext by address, so setting or clearing
+    // U_ext only rewrites that buffer (zeros when cleared): no graph goes stale.
+    BatchScf& b = s->s;
+    const long long NT = (long long)b.N * b.T;
+    NEGF_HIP_TRY(hipStreamSynchronize(b.st));
     if (!U_ext) {
-        s->s.has_ext = false;
+        int rc = fill_device(b.d_Uext.as<double>(), NT, 0.0, b.st);
+        if (rc) return rc;
+        NEGF_HIP_TRY(hipStreamSynchronize(b.st));
+        b.has_ext = false;
         return NEGF_OK;
     }
-    NEGF_HIP_TRY(hipMemcpy(s->s.d_Uext.p, U_ext, sizeof(double) * s->s.N * s->s.T, hipMemcpyHostToDevice));
-    s->s.has_ext = true;
+    for (long long i = 0; i < NT; ++i)
+        if (!std::isfinite(U_ext[i])) {
+            set_error("U_ext[%lld] is not finite", i);
+            return NEGF_EARG;
+        }
+    NEGF_HIP_TRY(hipMemcpy(b.d_Uext.p, U_ext, sizeof(double) * NT, hipMemcpyHostToDevice));
+    b.has_ext = true;
     return NEGF_OK;
 }
 

@@ -241,6 +241,11 @@ class BatchScf:
             pass
 
     def set_external_potential(self, U_ext):
+        """U_ext: [T][N] in eV, or None to clear it (zeros)."""
+        if U_ext is None:
+            self._U_ext = None
+            _lib.check(self.lib.negf_batch_scf_set_external_potential(self.h, None))
+            return
         self._U_ext = np.ascontiguousarray(U_ext, np.float64).reshape(self.T * self.N)
         _lib.check(self.lib.negf_batch_scf_set_external_potential(self.h, _dp(self._U_ext)))
 
@@ -256,8 +261,14 @@ class BatchScf:
     def use_graph(self, on=True):
         _lib.check(self.lib.negf_batch_scf_use_graph(self.h, int(on)))
 
-    def run(self, V_gate):
+    def _gates(self, V_gate):
         V = np.ascontiguousarray(V_gate, np.float64)
+        if V.size != self.T:
+            raise ValueError(f"V_gate needs one value per tube ({self.T}), got {V.size}")
+        return V.reshape(self.T)
+
+    def run(self, V_gate):
+        V = self._gates(V_gate)
         U = np.zeros(self.T * self.N)
         Q = np.zeros(self.T * self.N)
         it = np.zeros(self.T, np.int32)
@@ -286,7 +297,7 @@ class BatchScf:
         self._surrogate = surrogate
 
     def run_surrogate(self, V_gate, seed_iters=6, refresh_iter=0):
-        V = np.ascontiguousarray(V_gate, np.float64)
+        V = self._gates(V_gate)
         U = np.zeros(self.T * self.N)
         Q = np.zeros(self.T * self.N)
         it = np.zeros(self.T, np.int32)

@@ -167,32 +167,3 @@ def negf_pairs(batch, U, device="cuda"):
         dQ[s0:s0 + n] = buf.view(T, N)[:n]
     torch.cuda.synchronize()
     return dU, dQ
-
-
-def _main():
-    """python -m electrostatic_amd.surrogate [--batch B] [--sites N] [--reps R]:
-    times the forward pass and the tangent of an (untrained) surrogate on
-    random smooth potentials, for profiling the BF16 MFMA kernels."""
-    import argparse
-    import time
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--batch", type=int, default=128)
-    ap.add_argument("--sites", type=int, default=2048)
-    ap.add_argument("--reps", type=int, default=10)
-    a = ap.parse_args()
-    sg = Surrogate(a.sites, seed=1)
-    U = torch.as_tensor(sample_potentials(a.batch, a.sites, seed=7), device="cuda")
-    ones = torch.ones_like(U)
-    for name, fn in (("forward", lambda: sg.forward(U)), ("tangent", lambda: sg.jvp(U, ones))):
-        fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(a.reps):
-            fn()
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / a.reps
-        print(f"{name}: {ms:.3f} ms for {a.batch} x {a.sites} sites", flush=True)
-
-
-if __name__ == "__main__":
-    _main()

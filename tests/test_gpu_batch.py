@@ -188,3 +188,78 @@ def test_surrogate_preconditioned_scf_matches_plain(dev):
     scf.close()
     sg.close()
     batch.close()
+
+
+def _small_scf(dev, T=3, cells=20):
+    tubes = [negf.Tube(num_unitcells=cells, num_modes=1, contact_mu=(-1.0, -1.0),
+                       eq_integration_pts=(24, 24, 32), noneq_integration_pts=(48,),
+                       E_zPlus_imag=1e-5, E_valence_min=-10.0, E_pole_max=3.0) for _ in range(T)]
+    batch = negf.Batch(tubes)
+    scf = negf.BatchScf(batch, Broyden_fraction=0.1, Broyden_max_norm=1e-8,
+                        Broyden_threshold_maxstep=200, eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+    return tubes, batch, scf
+
+
+def test_batched_scf_graph_recaptured_after_batch_rebuild(dev):
+    # a rebuild of the batch's plans from outside the SCF (segment length,
+    # autotune) moves their buffers; the next graph replay must use the new ones
+    tubes, batch, scf = _small_scf(dev)
+    T, N = len(tubes), tubes[0].N
+    U_ext = np.stack([gaussian_bumps(N, seed=60 + t, amp=0.3) for t in range(T)])
+    V = np.array([0.0, 0.4, 0.9])
+    scf.set_external_potential(U_ext)
+    scf.set_input_charge(np.zeros((T, N)))
+    U_g1, _, it1, _, _ = scf.run(V)
+    batch.set_segment_length(32)          # larger workspace: plan buffers reallocated
+    scf.set_input_charge(np.zeros((T, N)))
+    U_g2, _, it2, nrm2, _ = scf.run(V)
+    scf.use_graph(False)
+    scf.set_input_charge(np.zeros((T, N)))
+    U_e, _, it_e, _, _ = scf.run(V)
+    assert np.all(nrm2 <= 1e-8)
+    assert np.abs(U_g2 - U_e).max() < 1e-10
+    assert np.abs(U_g1 - U_e).max() < 1e-10
+    assert list(it2) == list(it_e)
+    dQ = torch.zeros(int(batch.offsets[-1]), dtype=torch.float64, device=dev)
+    batch.autotune(dQ.data_ptr())
+    scf.use_graph(True)
+    scf.set_input_charge(np.zeros((T, N)))
+    U_g3, _, _, _, _ = scf.run(V)
+    assert np.abs(U_g3 - U_e).max() < 1e-10
+    scf.close()
+    batch.close()
+
+
+def test_batched_scf_external_potential_change_between_graph_runs(dev):
+    tubes, batch, scf = _small_scf(dev)
+    T, N = len(tubes), tubes[0].N
+    V = np.array([0.1, 0.5, 0.7])
+    results = {}
+    for graph in (True, False):
+        scf.use_graph(graph)
+        out = []
+        for U_ext in (None, np.stack([gaussian_bumps(N, seed=70 + t, amp=0.3) for t in range(T)]), None):
+            scf.set_external_potential(U_ext)
+            scf.set_input_charge(np.zeros((T, N)))
+            out.append(scf.run(V)[0])
+        results[graph] = out
+    for a, b in zip(results[True], results[False]):
+        assert np.abs(a - b).max() < 1e-10
+    # the runs with and without U_ext differ, and clearing U_ext restores the first
+    assert np.abs(results[True][0] - results[True][1]).max() > 1e-3
+    assert np.abs(results[True][0] - results[True][2]).max() < 1e-10
+    scf.close()
+    batch.close()
+
+
+def test_batched_scf_rejects_bad_host_inputs(dev):
+    tubes, batch, scf = _small_scf(dev)
+    T, N = len(tubes), tubes[0].N
+    bad = np.zeros((T, N))
+    bad[1, 3] = np.nan
+    with pytest.raises(Exception):
+        scf.set_external_potential(bad)
+    with pytest.raises(ValueError):
+        scf.run(0.2)                      # one gate value for three tubes
+    scf.close()
+    batch.close()
