@@ -9,8 +9,10 @@ smooth potential in [-0.5, 0.5] eV; output: the energy-integrated site charge
 the timed region starts.
 
 Prints one JSON line with the throughput (energy*site/s), the RGF kernel's
-roofline figures (FP64 flops per launch over its hipEvent-timed duration) and
-a CPU baseline (the oracle's serial RGF on the same arrays, one thread).
+roofline figures (FP64 flops per launch over its hipEvent-timed duration),
+the parity of the timed charge against the oracle on the same arrays, and a
+CPU baseline (the oracle's serial RGF on the same arrays: one thread, and
+energy chunks over every host core).
 """
 import argparse
 import ctypes as C
@@ -27,32 +29,63 @@ sys.path.insert(0, ROOT)
 METRIC = "RGF energy·site/s (% FP64 peak); SCF bias points/s surrogate vs plain"
 FLOP_PER_ENERGY_SITE = 31          # algorithmic FP64 flops (add/mul/div = 1, FMA = 2)
 FP64_PEAK_TFLOPS = 78.6            # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
+HOT_PATH_KERNELS = "transfer_kernel+scan_kernel+sweep_kernel (RGF hot path)"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rgf_seg_batched_traffic.json")
 
 
-def cpu_baseline(wl, max_seconds=30.0):
-    """Oracle (serial host RGF per energy, reference loop order) on the full
-    workload with one thread; falls back to a prefix of the energy list if the
-    host is too slow."""
+def host_cpu():
+    """CPU model (from /proc/cpuinfo) and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return model, cores
+
+
+def cpu_baseline(wl, max_seconds=15.0):
+    """The oracle (serial host RGF per energy, the reference's loop order) on
+    the full workload: once on one thread (the reference's per-rank
+    behaviour, USE_OMP=FALSE) on at most ~max_seconds of work, and once with
+    the energy list split over every core this process may use.  Returns the
+    baseline record and the all-core charge, which is also the parity
+    reference of the GPU result."""
     from oracle import oracle as O
     E, coef = wl["E"], wl["coef"]
     n = E.size
-    # time a small prefix to size the sample
+    args = (wl["U"], wl["t_even"], wl["t_odd"], wl["Uc"], wl["t_even"], wl["t_odd"])
+    model, cores = host_cpu()
+    # all cores, full workload
+    t0 = time.perf_counter()
+    ref, _ = O.accumulate_diagG_par(*args, E, coef, cores)
+    dt_all = time.perf_counter() - t0
+    # one thread: size the sample from a 256-energy prefix
     k0 = min(256, n)
     t0 = time.perf_counter()
-    O.accumulate_diagG(wl["U"], wl["t_even"], wl["t_odd"], wl["Uc"], wl["t_even"],
-                       wl["t_odd"], E[:k0], coef[:k0])
+    O.accumulate_diagG(*args, E[:k0], coef[:k0])
     dt0 = time.perf_counter() - t0
     k = n if dt0 * n / k0 <= max_seconds else max(k0, int(k0 * max_seconds / dt0))
     t0 = time.perf_counter()
-    O.accumulate_diagG(wl["U"], wl["t_even"], wl["t_odd"], wl["Uc"], wl["t_even"],
-                       wl["t_odd"], E[:k], coef[:k])
-    dt = time.perf_counter() - t0
-    rate = k * wl["N"] / dt
-    sample = (f"{k} of {n} energies x {wl['N']} sites (serial host RGF per energy, "
-              f"oracle restatement of the reference loop order), {dt:.2f} s")
-    return {"value": rate, "unit": "energy·site/s", "cores": 1, "kind": "port",
-            "sample": sample}
+    O.accumulate_diagG(*args, E[:k], coef[:k])
+    dt1 = time.perf_counter() - t0
+    rate_all = n * wl["N"] / dt_all
+    rate_1 = k * wl["N"] / dt1
+    sample = (f"full workload {n} energies x {wl['N']} sites on {cores} threads "
+              f"(energy chunks, OpenMP) in {dt_all:.2f} s; one thread: {k} of {n} energies "
+              f"in {dt1:.2f} s (serial host RGF per energy, oracle restatement of the "
+              f"reference loop order)")
+    rec = {"value": rate_all, "unit": "energy·site/s", "cores": cores, "kind": "port",
+           "sample": sample, "value_1core": rate_1, "cpu_model": model, "nproc": cores}
+    return rec, ref
 
 
 def scf_sweep(with_cpu=True):
@@ -235,6 +268,28 @@ def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
     return out
 
 
+def surrogate_kernels_leg(dev, B=128, N=2048, reps=10):
+    """Forward pass and tangent of an (untrained) surrogate on B seeded
+    smooth potentials of N sites, reps times each: the BF16 MFMA
+    convolution kernels in isolation (for kernel traces and counters)."""
+    import torch
+    from electrostatic_amd.surrogate import Surrogate, sample_potentials
+    sg = Surrogate(N, seed=1)
+    U = torch.as_tensor(sample_potentials(B, N, seed=7), device=dev)
+    ones = torch.ones_like(U)
+    out = {"workload": f"{B} samples x {N} sites, 6 conv layers (3->64, 4 x 64->64, 64->1), k=9"}
+    for name, fn in (("forward", lambda: sg.forward(U)), ("tangent", lambda: sg.jvp(U, ones))):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        out[f"ms_{name}"] = (time.perf_counter() - t0) * 1e3 / reps
+    sg.close()
+    return out
+
+
 def block_leg(steps=3, with_cpu=True):
     """Dense-block chain: atomistic (16,0) tube in blocks of two rings (32
     orbitals), 2000 blocks, 1024 energies (512 on a contour arc, 512 on the
@@ -245,9 +300,9 @@ def block_leg(steps=3, with_cpu=True):
     count of the same sweep (complex T, a second inverse for G: 56 m^3) is
     reported beside it."""
     from electrostatic_amd import negf
-    from oracle import block_oracle as B
+    from electrostatic_amd.workloads import zigzag_ring_blocks
     nb, nE, m = 2000, 1024, 32
-    H0, T = B.zigzag_blocks(16, 2.5)
+    H0, T = zigzag_ring_blocks(16, 2.5)
     rng = np.random.default_rng(4)
     v = rng.normal(0.0, 0.05, (nb, m)) + 0.2 * np.sin(np.linspace(0, np.pi, nb))[:, None]
     E = np.concatenate([np.linspace(-3, 3, nE // 2) + 1e-5j,
@@ -274,11 +329,13 @@ def block_leg(steps=3, with_cpu=True):
            "ms_per_evaluation": wall, "ms_sancho_rubio": float(np.median(sr_ms)), "ms_block_rgf": rg,
            "sancho_rubio_iterations_max": int(it.max()),
            "energy_blocks_per_s": nb * nE / (wall * 1e-3),
+           "flop_per_energy_block": 36 * m ** 3,
            "rgf_tflops": flops / (rg * 1e-3) / 1e12,
            "rgf_frac_fp64_peak": flops / (rg * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
-           "rgf_frac_fp64_peak_general_block_count": 56 * m ** 3 * nb * nE / (rg * 1e-3) / 1e12
-                                                     / FP64_PEAK_TFLOPS}
+           "note": "fraction at the executed 36 m^3 flops per energy-block (real H0 and T); "
+                   "a general complex block would need 56 m^3"}
     if with_cpu:
+        from oracle import block_oracle as B
         # numpy oracle on 4 energies of the same chain
         sample = E[::nE // 4][:4]
         t0 = time.perf_counter()
@@ -305,6 +362,8 @@ def main():
                     help="skip the dense-block (32-orbital) chain leg")
     ap.add_argument("--no-scf", action="store_true",
                     help="skip the secondary SCF gate-sweep measurement")
+    ap.add_argument("--surrogate-kernels", action="store_true",
+                    help="also time the surrogate's forward and tangent passes alone")
     ap.add_argument("--write-diag-g", action="store_true",
                     help="also materialise diag G^R ([energy][site], 512 MiB)")
     args = ap.parse_args()
@@ -381,7 +440,37 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE) and not args.write_diag_g:
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tr = json.load(f)
+        traffic = tr.get("hbm_bytes_per_launch")
+
+    if args.write_diag_g:
+        # the G^R-writing variant is bound by HBM: 16 B per energy-site written
+        gbytes = 16.0 * energy_sites
+        roof = {
+            "bound": "hbm",
+            "kernel": "RGF hot path with diag G^R written ([energy][site] complex FP64)",
+            "bytes_per_energy_site": 16,
+            "kernel_ms": kernel_ms,
+            "achieved": gbytes / (kernel_ms * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": gbytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "fp64_tflops": achieved,
+        }
+    else:
+        roof = {
+            "bound": "fp64_valu",
+            "compute": "FP64 VALU, compute-bound (MI355X FP64 peak 78.6 TFLOP/s)",
+            "kernel": HOT_PATH_KERNELS,
+            "flop_per_energy_site": FLOP_PER_ENERGY_SITE,
+            "kernel_ms": kernel_ms,
+            "achieved": achieved,
+            "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS,
+            "traffic": traffic,
+        }
 
     out = {
         "metric": METRIC,
@@ -407,21 +496,30 @@ def main():
             "energy_sites": energy_sites,
             "segment_length": args.segment_length or 128,
         },
-        "roofline": {
-            "bound": "mfma",
-            "compute": "FP64 VALU (MI355X FP64 peak 78.6 TFLOP/s, vector = matrix)",
-            "kernel": "transfer_kernel+scan_kernel+sweep_kernel (RGF hot path)",
-            "flop_per_energy_site": FLOP_PER_ENERGY_SITE,
-            "kernel_ms": kernel_ms,
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": traffic,
-        },
+        "roofline": roof,
     }
+    # parity of the timed result against the oracle on the same arrays
+    cpu_rec, ref = cpu_baseline(wl)
+    err = float(np.abs(q - ref).max() / np.abs(ref).max())
+    out["max_rel_err_vs_oracle"] = err
+    out["parity"] = {"max_rel_err_vs_oracle": err, "tolerance": 1e-10,
+                     "oracle": "oracle restatement of the reference RGF, full workload"}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl)
+        out["cpu_baseline"] = cpu_rec
+    if args.write_diag_g:
+        # spot check of the materialised G^R rows against the oracle
+        from oracle import oracle as O
+        ks = np.array([0, nE // 3, nE // 2, nE - 1])
+        _, dg_ref, _ = O.accumulate_diagG(wl["U"], wl["t_even"], wl["t_odd"], wl["Uc"],
+                                          wl["t_even"], wl["t_odd"], wl["E"][ks],
+                                          wl["coef"][ks], want_diag=True)
+        g = dG.view(nE, N, 2)[torch.as_tensor(ks, device=dev)].cpu().numpy()
+        g = g[..., 0] + 1j * g[..., 1]
+        out["diag_G_max_rel_err_vs_oracle"] = float(np.abs(g - dg_ref).max() / np.abs(dg_ref).max())
+    if err > 1e-10:
+        print(json.dumps(out), flush=True)
+        print(f"parity failure: max rel err {err:.3e} vs oracle", file=sys.stderr)
+        sys.exit(1)
     lib.negf_rgf_plan_destroy(plan)
     if not args.no_scf:
         out["scf"] = scf_sweep(with_cpu=not args.no_cpu_baseline)
@@ -429,6 +527,8 @@ def main():
         out["batch"] = batch_leg(dev)
     if not args.no_block:
         out["block"] = block_leg(with_cpu=not args.no_cpu_baseline)
+    if args.surrogate_kernels:
+        out["surrogate_kernels"] = surrogate_kernels_leg(dev)
     if not args.no_surrogate:
         out["surrogate_scf"] = surrogate_leg(dev, with_cpu=not args.no_cpu_baseline)
     print(json.dumps(out), flush=True)

@@ -97,3 +97,24 @@ def tube_batch(n_tubes=64, lo=500, hi=20000, seed=3, bias_max=0.5):
     N = np.round(np.exp(rng.uniform(np.log(lo), np.log(hi), n_tubes))).astype(int)
     bias = rng.uniform(0, bias_max, n_tubes)
     return N, bias
+
+
+def zigzag_ring_blocks(m=16, gamma=2.5):
+    """Nearest-neighbour pi-orbital Hamiltonian of an (m,0) zigzag tube cut
+    into blocks of two rings (2m orbitals), hoppings -gamma.
+
+    Within a block, atom j of the first ring bonds to atoms j and j-1 of the
+    second ring (the zigzag bonds).  Between blocks, atom j of a block's
+    second ring bonds axially to one atom of the next block's first ring;
+    numbering the atoms of every other block by the mirrored angle makes
+    that partner atom -j mod m, so every block has the same on-site matrix
+    H0 and the same coupling T = H_{b,b+1}.  Returns (H0, T), real 2m x 2m.
+    """
+    zig = np.eye(m) + np.roll(np.eye(m), -1, axis=1)     # j -> j and j-1
+    H0 = np.zeros((2 * m, 2 * m))
+    H0[:m, m:] = -gamma * zig
+    H0[m:, :m] = -gamma * zig.T
+    T = np.zeros((2 * m, 2 * m))
+    j = np.arange(m)
+    T[m + j, (-j) % m] = -gamma
+    return H0, T

@@ -820,6 +820,36 @@ int orc_accumulate_diagG(int N, const double *U, double t_even, double t_odd,
     return st;
 }
 
+/* The same accumulation with the energy list split into nthreads
+ * contiguous chunks, one OpenMP thread each (the reference's loop order
+ * inside a chunk); chunk sums are added in chunk order, so the result is
+ * reproducible for a given nthreads.  Used to time the CPU baseline on all
+ * host cores. */
+int orc_accumulate_diagG_par(int nthreads, int N, const double *U,
+                             double t_even, double t_odd, const double Uc[2],
+                             double lead_beta, double lead_gamma, int nE,
+                             const orc_c *E, const orc_c *coef, double *acc_re)
+{
+    if (nthreads < 1) nthreads = 1;
+    std::vector<std::vector<double>> part(nthreads, std::vector<double>(N, 0.));
+    std::vector<int> st(nthreads, 0);
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1)
+    for (int c = 0; c < nthreads; ++c) {
+        const int k0 = int((long long)nE * c / nthreads);
+        const int k1 = int((long long)nE * (c + 1) / nthreads);
+        if (k1 > k0)
+            st[c] = orc_accumulate_diagG(N, U, t_even, t_odd, Uc, lead_beta,
+                                         lead_gamma, k1 - k0, E + k0, coef + k0,
+                                         part[c].data(), nullptr);
+    }
+    int s = 0;
+    for (int c = 0; c < nthreads; ++c) {
+        s |= st[c];
+        for (int n = 0; n < N; ++n) acc_re[n] += part[c][n];
+    }
+    return s;
+}
+
 /* Generic checker for the non-equilibrium integrand: acc_re[n] +=
  * coefL_k * Gamma_L |G_n0|^2 + coefR_k * Gamma_R |G_n,N-1|^2 with the
  * reference's O(N^2) walk order (NEGF_Common.cpp:3411-3428). */

@@ -107,6 +107,12 @@ int orc_accumulate_diagG(int N, const double *U, double t_even, double t_odd,
                          const double Uc[2], double lead_beta,
                          double lead_gamma, int nE, const orc_c *E,
                          const orc_c *coef, double *acc_re, orc_c *diagG);
+/* Same sum over nthreads OpenMP threads (contiguous energy chunks, chunk
+ * sums added in order). */
+int orc_accumulate_diagG_par(int nthreads, int N, const double *U,
+                             double t_even, double t_odd, const double Uc[2],
+                             double lead_beta, double lead_gamma, int nE,
+                             const orc_c *E, const orc_c *coef, double *acc_re);
 int orc_accumulate_noneq(int N, const double *U, double t_even, double t_odd,
                          const double Uc[2], double lead_beta,
                          double lead_gamma, int nE, const orc_c *E,

@@ -140,6 +140,10 @@ def _declare(L):
                                        dp, C.c_double, C.c_double, C.c_int,
                                        cp, cp, dp, cp]
     L.orc_accumulate_diagG.restype = C.c_int
+    L.orc_accumulate_diagG_par.argtypes = [C.c_int, C.c_int, dp, C.c_double, C.c_double,
+                                           dp, C.c_double, C.c_double, C.c_int,
+                                           C.c_void_p, C.c_void_p, dp]
+    L.orc_accumulate_diagG_par.restype = C.c_int
     L.orc_accumulate_noneq.argtypes = [C.c_int, dp, C.c_double, C.c_double,
                                        dp, C.c_double, C.c_double, C.c_int,
                                        cp, dp, dp, dp]
@@ -352,6 +356,19 @@ def accumulate_diagG(U, t_even, t_odd, Uc, lead_beta, lead_gamma, E, coef,
                                     _cp(coef), _dp(acc),
                                     _cp(dg) if want_diag else None)
     return (acc, dg, st) if want_diag else (acc, st)
+
+
+def accumulate_diagG_par(U, t_even, t_odd, Uc, lead_beta, lead_gamma, E, coef, nthreads):
+    """accumulate_diagG over nthreads OpenMP threads (energy chunks)."""
+    U = np.ascontiguousarray(U, np.float64)
+    Uc = np.ascontiguousarray(Uc, np.float64)
+    E = _carr(E)
+    coef = _carr(coef)
+    acc = np.zeros(U.size)
+    st = lib().orc_accumulate_diagG_par(int(nthreads), U.size, _dp(U), t_even, t_odd, _dp(Uc),
+                                        lead_beta, lead_gamma, E.size, _cp(E), _cp(coef),
+                                        _dp(acc))
+    return acc, st
 
 
 def accumulate_noneq(U, t_even, t_odd, Uc, lead_beta, lead_gamma, E, cL, cR):

@@ -292,3 +292,56 @@ def test_scaled_and_unscaled_noneq_sweeps_agree():
         assert rel_err(got, ref) < 1e-10
         accs.append(got)
     assert rel_err(accs[1], accs[0]) < 1e-11
+
+
+def test_full_batched_workload_matches_oracle():
+    """The benchmark's own workload at full size (8192 energies x 4096
+    sites: reference contour arc with Fermi-weighted Gauss-Legendre
+    coefficients plus the real axis at eta = 1e-5) through the plan entry
+    points the benchmark times, against the oracle on all host cores."""
+    import ctypes as C
+    import os
+    from electrostatic_amd.workloads import batched_rgf_workload
+    lib = _lib.load()
+    wl = batched_rgf_workload()
+    N, nE = wl["N"], wl["E"].size
+    p = chain_params(N, wl["t_even"], wl["t_odd"], wl["Uc"])
+    dU, dE, dc = dev_array(wl["U"]), dev_array(wl["E"]), dev_array(wl["coef"])
+    acc = dev_zeros(N)
+    plan = C.c_void_p()
+    _lib.check(lib.negf_rgf_plan_create(p, ptr(dU), nE, ptr(dE), ptr(dc), 0, C.byref(plan)))
+    try:
+        for _ in range(2):   # a second execute reuses the plan's workspace
+            _lib.check(lib.negf_rgf_plan_execute(plan, ptr(acc), None, None))
+        _lib.check(lib.negf_rgf_plan_status(plan, None))
+    finally:
+        lib.negf_rgf_plan_destroy(plan)
+    got = to_np(acc)
+    ncpu = len(os.sched_getaffinity(0))
+    ref, st = O.accumulate_diagG_par(wl["U"], wl["t_even"], wl["t_odd"], wl["Uc"],
+                                     wl["t_even"], wl["t_odd"], wl["E"], wl["coef"], ncpu)
+    assert st == 0
+    assert rel_err(got, ref) < 1e-10
+
+
+@pytest.mark.parametrize("eta,flat", [(1e-5, True), (1e-5, False), (1e-8, True)])
+def test_band_edge_energies_match_oracle(eta, flat):
+    """Real-axis energies within a few eta of the four band edges
+    +-(gamma - beta), +-(gamma + beta) of the two-hopping chain, where the
+    1-D density of states diverges and the recurrences barely decay; N = 4096.
+    eta = 1e-8 is the reference's default E_zPlus (NEGF_Common.H:386)."""
+    N = 4096
+    te, to = BETA17, GAMMA
+    U = np.zeros(N) if flat else smooth_potential(N, seed=9, lo=-0.05, hi=0.05)
+    edges = [-(to + te), -(to - te), to - te, to + te]
+    offs = np.array([-30, -10, -3, -1, 0, 1, 3, 10, 30]) * eta
+    E = np.array([e + o for e in edges for o in offs]) + 1j * eta
+    rng = np.random.default_rng(4)
+    coef = rng.normal(size=E.size) + 1j * rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    acc_ref, G_ref, st = O.accumulate_diagG(U, te, to, Uc, te, to, E, coef, want_diag=True)
+    acc, G = run_diag(U, te, to, Uc, E, coef)
+    assert np.all(np.isfinite(acc))
+    assert rel_err(acc, acc_ref) < 1e-10
+    big = np.abs(G_ref) > 1e-6 * np.abs(G_ref).max()
+    assert (np.abs(G - G_ref)[big] / np.abs(G_ref)[big]).max() < 1e-9
