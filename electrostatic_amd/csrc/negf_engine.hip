@@ -346,12 +346,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         NEGF_HIP_TRY(hipEventRecord(ev_start_[ev_used_], st));
     }
     hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((nitems_ + 3) / 4), dim3(256), 0, st, lp);
-    if (max_P_ <= 4 * SCAN_WAVES)
-        hipLaunchKernelGGL(scan_kernel<4>, dim3(nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
-    else if (max_P_ > 8 * SCAN_WAVES)
-        hipLaunchKernelGGL((scan_kernel<0, 2 * SCAN_WAVES>), dim3(nsblk_), dim3(128 * SCAN_WAVES), 0, st, lp, nsblk_);
-    else
-        hipLaunchKernelGGL(scan_kernel<0>, dim3(nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
+    hipLaunchKernelGGL(scan_kernel, dim3(2 * nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
     for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
         if (fast_n_[ns] == 0) continue;
         WLaunch lf = lp;

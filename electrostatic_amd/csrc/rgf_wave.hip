@@ -29,9 +29,10 @@
 // (64-chain group, segment) item.  Three kernels:
 //   transfer_kernel  2x2 transfer matrix of each segment (both basis columns),
 //                    renormalised by exact powers of two;
-//   scan_kernel      Hillis-Steele prefix and suffix products over the P
-//                    segments of a chain (lanes = segments here) give each
-//                    segment its exact left state u and right state v, and D;
+//   scan_kernel      walks the P segment matrices of a chain (u forward, v
+//                    backward, one matrix-vector product per segment) and
+//                    gives each segment its left state u and right state v,
+//                    and D;
 //   sweep_kernel     backward v over the segment with register checkpoints
 //                    every K sites, then per K-site sub-block: v recomputed
 //                    into registers, u forward, G and the charge contribution
@@ -427,8 +428,8 @@ __device__ __forceinline__ void bstep(const SCoefs& k, double U, double2& x, dou
 
 // Both basis columns through n sites of the staged segment (the first at an
 // even site), renormalised every K sites.
-template <int K, class KC>
-__device__ __forceinline__ void transfer_loop(const KC& k, const double* us, int n, double2& m00, double2& m01,
+template <int K, class KC, class UP>
+__device__ __forceinline__ void transfer_loop(const KC& k, UP us, int n, double2& m00, double2& m01,
                                               double2& m10, double2& m11, int& eT)
 {
     for (int q0 = 0; q0 < n; q0 += K) {
@@ -448,26 +449,6 @@ __device__ __forceinline__ void transfer_loop(const KC& k, const double* us, int
             }
         }
         renorm_mat(m00, m01, m10, m11, eT);
-    }
-}
-
-// Two independent products over the halves [0, h) and [h, 2h) of the staged
-// segment (h a multiple of K, the first site even), interleaved site by site.
-template <int K, class KC, class UP>
-__device__ __forceinline__ void transfer_loop2(const KC& k, UP us, int h, double2& m00, double2& m01,
-                                               double2& m10, double2& m11, int& eA, double2& b00, double2& b01,
-                                               double2& b10, double2& b11, int& eB)
-{
-    for (int q0 = 0; q0 < h; q0 += K) {
-#pragma unroll
-        for (int i = 0; i < K; i += 2) {
-            mstep<0>(k, us[q0 + i], m00, m01, m10, m11);
-            mstep<0>(k, us[h + q0 + i], b00, b01, b10, b11);
-            mstep<1>(k, us[q0 + i + 1], m00, m01, m10, m11);
-            mstep<1>(k, us[h + q0 + i + 1], b00, b01, b10, b11);
-        }
-        renorm_mat(m00, m01, m10, m11, eA);
-        renorm_mat(b00, b01, b10, b11, eB);
     }
 }
 
@@ -501,32 +482,12 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
         // the rows return to u with s_L = C ai_{hi-1} and s_{L-1} = C
         const SCoefs ks = make_scoefs(E, ch.t_even, tube.t_odd);
         const double a0 = ((lo - 1) & 1) ? ks.ai[1] : ks.ai[0];
-        if (hi - lo == tube.L && tube.L % (2 * K) == 0) {
-            // The scaled coefficients do not depend on where the scales
-            // start, so the two halves of the segment are independent
-            // products (twice the independent FMA chains per lane), joined
-            // by one 2x2 product; column 1 then takes ai_{lo-1}.
-            double2 b00 = make_double2(1.0, 0.0), b01 = make_double2(0.0, 0.0);
-            double2 b10 = make_double2(0.0, 0.0), b11 = make_double2(1.0, 0.0);
-            int eB = 0;
-            // U is wave-uniform and read-only here: through the constant
-            // address space it arrives by scalar loads into SGPRs (U from the
-            // LDS copy took 64 VGPRs for the two halves and spilled 24 B per
-            // lane to scratch)
-            typedef const __attribute__((address_space(4))) double cdbl;
-            const cdbl* ug = (const cdbl*)(uintptr_t)(tube.U + lo);
-            transfer_loop2<K>(ks, ug, tube.L / 2, m00, m01, m10, m11, eT, b00, b01, b10, b11, eB);
-            double2 r00, r01, r10, r11;
-            mat_mul(b00, b01, b10, b11, m00, m01, m10, m11, r00, r01, r10, r11);
-            m00 = r00;
-            m01 = cscale(a0, r01);
-            m10 = r10;
-            m11 = cscale(a0, r11);
-            eT += eB;
-        } else {
-            m11 = make_double2(a0, 0.0);
-            transfer_loop<K>(ks, us, hi - lo, m00, m01, m10, m11, eT);
-        }
+        m11 = make_double2(a0, 0.0);
+        // U is wave-uniform and read-only here: through the constant address
+        // space it arrives by scalar loads into SGPRs instead of LDS reads
+        typedef const __attribute__((address_space(4))) double cdbl;
+        const cdbl* ug = (const cdbl*)(uintptr_t)(tube.U + lo);
+        transfer_loop<K>(ks, ug, hi - lo, m00, m01, m10, m11, eT);
         double mC;
         int eC;
         scale_prod(ks, lo & 1, hi - lo - 1, mC, eC);
@@ -563,15 +524,31 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
 //      (product of the per-site determinants c_n);
 //      V_N = (v_N, v_{N-1}) = (Sigma_R/t_{N-1}, 1)
 //   D = t_{N-1} u_N - Sigma_R u_{N-1}.
-// One workgroup of SCAN_WAVES wavefronts per 64-chain group, lane = chain,
-// so every Tm read and St write is a coalesced row.  Wave w owns a contiguous
-// range of segments: it multiplies its transfer matrices (phase A), the
-// range products are exchanged through LDS and each wave forms its entry
-// states from the products of the ranges before (u) and after (v) it
-// (phase B), then it runs both states through its own range writing the
-// per-segment states (phase C).  The sequential depth is about 3 P /
-// SCAN_WAVES matrix steps instead of 2 P.
-#define SCAN_WAVES 8
+// One workgroup of SCAN_WAVES wavefronts per (64-chain group, walk), lane =
+// chain, so every Tm read and St write is a coalesced row.  The u walk applies
+// T_0, T_1, ... to U_0, the v walk T_{P-1}^{-1}, T_{P-2}^{-1}, ... to V_N; each
+// writes the entry state of every segment it passes.  A walk is cut into
+// SCAN_WAVES contiguous ranges of segments, one per wave, taken in turn (the
+// state is handed on through LDS).  Every wave requests the matrices of its
+// range one turn ahead of its own (the first wave when the kernel starts), so
+// the loads of the next range arrive while the current one is walked and the
+// first range's loads do not queue behind everybody else's.  The two walks of
+// a chain group are separate workgroups, so neither waits for the other.
+// The walks use matrix-vector products only: near a band edge a segment
+// matrix is close to a defective matrix whose entries are of the order of the
+// segment length while the state is close to its eigenvector, and a product
+// of two such matrices rounds at |T_a| |T_b| eps in directions that the later
+// walk does not damp (about 1e-9 relative in G_nn 1e-5 eV from the outer band
+// edges of a 4096-site chain), where a matrix-vector product from the
+// normalised state rounds at |T| eps.
+// The state is renormalised every SCAN_RENORM steps only (off the dependency
+// chain of the other steps): the matrices' mantissas have real and imaginary
+// parts below 1 in magnitude, so a state grows by less than a factor 4 per
+// step, and the stored entry states are exact (mantissa, exponent) pairs
+// whether or not they were just renormalised.
+#define SCAN_WAVES 4
+#define SCAN_CACHE 8
+#define SCAN_RENORM 4
 
 struct Mat2 {
     double2 a, b, c, d;  // [[a, b], [c, d]]
@@ -590,16 +567,6 @@ __device__ __forceinline__ Mat2 load_T(const WLaunch& lp, long long item, int la
     return m;
 }
 
-// r = x * y
-__device__ __forceinline__ Mat2 mat2_mul(const Mat2& x, const Mat2& y)
-{
-    Mat2 r;
-    mat_mul(x.a, x.b, x.c, x.d, y.a, y.b, y.c, y.d, r.a, r.b, r.c, r.d);
-    r.e = x.e + y.e;
-    renorm_mat(r.a, r.b, r.c, r.d, r.e);
-    return r;
-}
-
 // (x, y) <- M (x, y)
 __device__ __forceinline__ void mat2_apply(const Mat2& m, double2& x, double2& y, int& ex)
 {
@@ -609,6 +576,26 @@ __device__ __forceinline__ void mat2_apply(const Mat2& m, double2& x, double2& y
     y = ny;
     ex += m.e;
     renorm(x, y, ex);
+}
+
+// (x, y) <- M (x, y) without renormalisation
+__device__ __forceinline__ void mat2_apply_raw(const Mat2& m, double2& x, double2& y, int& ex)
+{
+    const double2 nx = cadd(cmul(m.a, x), cmul(m.b, y));
+    const double2 ny = cadd(cmul(m.c, x), cmul(m.d, y));
+    x = nx;
+    y = ny;
+    ex += m.e;
+}
+
+// (x, y) <- M^{-1} (x, y) without renormalisation
+__device__ __forceinline__ void mat2_apply_inv_raw(const Mat2& m, double idet, double2& x, double2& y, int& ex)
+{
+    const double2 nx = cscale(idet, csub(cmul(m.d, x), cmul(m.b, y)));
+    const double2 ny = cscale(idet, csub(cmul(m.a, y), cmul(m.c, x)));
+    x = nx;
+    y = ny;
+    ex += m.e;
 }
 
 // (x, y) <- M^{-1} (x, y) = adj(M) (x, y) / det, idet = 1/det of the true matrix
@@ -622,19 +609,25 @@ __device__ __forceinline__ void mat2_apply_inv(const Mat2& m, double idet, doubl
     renorm(x, y, ex);
 }
 
-// CACHE > 0: every wave's range has at most CACHE segments (P <= CACHE *
-// W for every tube of the launch), and the transfer matrices stay in
-// registers from phase A to phase C instead of being read again.  W: waves
-// per chain group (16 for long chains, whose sequential depth is then about
-// 2 P / 16 + 16 matrix steps instead of 2 P / 8 + 8).
-template <int CACHE, int W = SCAN_WAVES>
-__global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
+// Loads of up to SCAN_CACHE transfer matrices, walk steps [i0, i0 + SCAN_CACHE)
+// of the walk (clamped to the last segment, so the loads are unconditional
+// and the compiler can count them exactly).
+template <class SEG>
+__device__ __forceinline__ void scan_load(const WLaunch& lp, long long item0, SEG seg, int i0, int P, int lane,
+                                          Mat2 (&buf)[SCAN_CACHE])
 {
-    __shared__ double xm[W][9][64];
+#pragma unroll
+    for (int q = 0; q < SCAN_CACHE; ++q) buf[q] = load_T(lp, item0 + seg(min(i0 + q, P - 1)), lane);
+}
+
+__global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int ngroups)
+{
+    __shared__ double hand[5][64];  // state handed from one range to the next
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int gi = blockIdx.x;
-    if (gi >= ngroups) return;  // uniform over the block
+    const int r = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // range, counted from the walk's start
+    const int gi = blockIdx.x >> 1;
+    const int dir = blockIdx.x & 1;  // 0: u walk, 1: v walk
+    if (gi >= ngroups) return;       // uniform over the block
     const int t = lp.sblk_tube[gi];
     const int g = lp.sblk_chain[gi];
     const WTube tube = lp.tubes[t];
@@ -649,149 +642,87 @@ __global__ __launch_bounds__(64 * W) void scan_kernel(WLaunch lp, int ngroups)
     const double2 SL = make_double2(ch.SLr, ch.SLi);
     const double2 SR = make_double2(ch.SRr, ch.SRi);
     const long long item0 = tube.item0 + (long long)g * P;
-    // segment range of each wave, and 1/det of a range [lo, hi) of sites
-    const int per = (P + W - 1) / W;
-    auto range_lo = [&](int k) { return min(P, k * per); };
-    auto inv_det = [&](int slo, int shi) {
-        // sites [slo*L, min(N, shi*L)); t_{lo-1} is an odd link (lo even)
-        const int hi = min(N, shi * L);
-        const int lo = min(N, slo * L);
-        if (lo >= hi) return 1.0;
-        const double t_hm1 = ((hi - 1) & 1) ? to : te;
-        return t_hm1 / to;
-    };
-    const int s0 = range_lo(w), s1 = range_lo(w + 1);
-
-    // phase A: M_w = T_{s1-1} ... T_{s0}
-    Mat2 M;
-    M.a = make_double2(1.0, 0.0);
-    M.b = make_double2(0.0, 0.0);
-    M.c = make_double2(0.0, 0.0);
-    M.d = make_double2(1.0, 0.0);
-    M.e = 0;
-    constexpr bool cached = CACHE > 0;
-    constexpr int NT = cached ? CACHE : 4;
-    Mat2 Tk[NT];
-    if (cached) {
+    const int per = (P + SCAN_WAVES - 1) / SCAN_WAVES;
+    const int r0 = min(P, r * per), cnt = min(P, r0 + per) - r0;  // walk steps [r0, r0 + cnt)
+    // segment of walk step i
+    auto seg = [dir, P](int i) { return dir == 0 ? i : P - 1 - i; };
+    Mat2 buf[SCAN_CACHE];
+    if (r == 0) scan_load(lp, item0, seg, r0, P, lane, buf);
+    const double idet_odd = te / to;  // 1/det T_s when hi - 1 is even
+    for (int k = 0; k < SCAN_WAVES; ++k) {
+        if (r == k + 1 && cnt > 0) scan_load(lp, item0, seg, r0, P, lane, buf);
+        if (r == k && cnt > 0) {
+            double2 x, y;
+            int ex = 0;
+            if (k == 0) {
+                if (dir == 0) {
+                    x = make_double2(1.0, 0.0);  // u_0
+                    y = cscale(1.0 / to, SL);    // u_{-1}; t_{-1} is an odd link
+                } else {
+                    x = cscale(1.0 / tau, SR);   // v_N
+                    y = make_double2(1.0, 0.0);  // v_{N-1}
+                }
+            } else {
+                x = make_double2(hand[0][lane], hand[1][lane]);
+                y = make_double2(hand[2][lane], hand[3][lane]);
+                ex = (int)hand[4][lane];
+            }
+            for (int c = 0; c < cnt; c += SCAN_CACHE) {
+                if (c > 0) scan_load(lp, item0, seg, r0 + c, P, lane, buf);
 #pragma unroll
-        for (int q = 0; q < NT; ++q)
-            if (s0 + q < s1) Tk[q] = load_T(lp, item0 + s0 + q, lane);
-        if (NT == 4 && s1 - s0 == 4) {
-            // product tree: depth two instead of a chain of four
-            const Mat2 p01 = mat2_mul(Tk[1], Tk[0]);
-            const Mat2 p23 = mat2_mul(Tk[3 % NT], Tk[2 % NT]);
-            M = mat2_mul(p23, p01);
-        } else {
-#pragma unroll
-            for (int q = 0; q < NT; ++q)
-                if (s0 + q < s1) M = mat2_mul(Tk[q], M);
+                for (int q = 0; q < SCAN_CACHE; ++q) {
+                    if (c + q < cnt) {
+                        const int s = seg(r0 + c + q);
+                        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
+                        if (dir == 0) {
+                            S[0 * 64] = x.x;  // (u_lo, u_{lo-1})
+                            S[1 * 64] = x.y;
+                            S[2 * 64] = y.x;
+                            S[3 * 64] = y.y;
+                            S[4 * 64] = (double)ex;
+                            mat2_apply_raw(buf[q], x, y, ex);
+                        } else {
+                            S[5 * 64] = y.x;  // (v_{hi-1}, v_hi)
+                            S[6 * 64] = y.y;
+                            S[7 * 64] = x.x;
+                            S[8 * 64] = x.y;
+                            S[9 * 64] = (double)ex;
+                            const int lo = min(N, s * L);
+                            const int hi = min(N, lo + L);
+                            // det T_s = t_{lo-1} / t_{hi-1}; t_{lo-1} is an odd link (lo even)
+                            if (lo < hi) mat2_apply_inv_raw(buf[q], ((hi - 1) & 1) ? 1.0 : idet_odd, x, y, ex);
+                        }
+                        if (q % SCAN_RENORM == SCAN_RENORM - 1) renorm(x, y, ex);
+                    }
+                }
+            }
+            hand[0][lane] = x.x;
+            hand[1][lane] = x.y;
+            hand[2][lane] = y.x;
+            hand[3][lane] = y.y;
+            hand[4][lane] = (double)ex;
         }
-    } else {
-        for (int c = s0; c < s1; c += 4) {
-            // issue the loads of up to four segments before the products
-#pragma unroll
-            for (int q = 0; q < NT; ++q)
-                if (c + q < s1) Tk[q] = load_T(lp, item0 + c + q, lane);
-#pragma unroll
-            for (int q = 0; q < NT; ++q)
-                if (c + q < s1) M = mat2_mul(Tk[q], M);
-        }
+        // the hand-off goes through LDS only: wait for the LDS writes, then
+        // meet (a full __syncthreads would also wait for the walk's stores)
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        __builtin_amdgcn_s_barrier();
     }
-    xm[w][0][lane] = M.a.x;
-    xm[w][1][lane] = M.a.y;
-    xm[w][2][lane] = M.b.x;
-    xm[w][3][lane] = M.b.y;
-    xm[w][4][lane] = M.c.x;
-    xm[w][5][lane] = M.c.y;
-    xm[w][6][lane] = M.d.x;
-    xm[w][7][lane] = M.d.y;
-    xm[w][8][lane] = (double)M.e;
-    __syncthreads();
-    auto get = [&](int k) {
-        Mat2 m;
-        m.a = make_double2(xm[k][0][lane], xm[k][1][lane]);
-        m.b = make_double2(xm[k][2][lane], xm[k][3][lane]);
-        m.c = make_double2(xm[k][4][lane], xm[k][5][lane]);
-        m.d = make_double2(xm[k][6][lane], xm[k][7][lane]);
-        m.e = (int)xm[k][8][lane];
-        return m;
-    };
-
-    // phase B: entry states of this wave's range
-    double2 ux = make_double2(1.0, 0.0);
-    double2 uy = cscale(1.0 / to, SL);  // u_{-1}; t_{-1} is an odd link
-    int ue = 0;
-    for (int k = 0; k < w; ++k) mat2_apply(get(k), ux, uy, ue);
-    double2 vx = cscale(1.0 / tau, SR);  // v_N
-    double2 vy = make_double2(1.0, 0.0);  // v_{N-1}
-    int ve = 0;
-    for (int k = W - 1; k > w; --k) {
-        const int k0 = range_lo(k), k1 = range_lo(k + 1);
-        if (k0 < k1) mat2_apply_inv(get(k), inv_det(k0, k1), vx, vy, ve);
-    }
-
-    // phase C: per-segment states through this wave's range (the u and v
-    // walks are independent; with the matrices in registers they advance in
-    // one loop)
-    auto u_step = [&](int s, const Mat2& Ts) {
-        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
-        S[0 * 64] = ux.x;
-        S[1 * 64] = ux.y;
-        S[2 * 64] = uy.x;
-        S[3 * 64] = uy.y;
-        S[4 * 64] = (double)ue;
-        mat2_apply(Ts, ux, uy, ue);
-    };
-    auto v_step = [&](int s, const Mat2& Ts) {
-        double* S = lp.St + (item0 + s) * 10 * 64 + lane;
-        S[5 * 64] = vy.x;  // (v_{hi-1}, v_hi)
-        S[6 * 64] = vy.y;
-        S[7 * 64] = vx.x;
-        S[8 * 64] = vx.y;
-        S[9 * 64] = (double)ve;
-        const int lo = min(N, s * L);
-        const int hi = min(N, lo + L);
-        if (lo < hi) mat2_apply_inv(Ts, inv_det(s, s + 1), vx, vy, ve);
-    };
-    if (cached) {
-#pragma unroll
-        for (int q = 0; q < NT; ++q) {
-            if (s0 + q < s1) u_step(s0 + q, Tk[q]);
-            if (s0 + NT - 1 - q < s1) v_step(s0 + NT - 1 - q, Tk[NT - 1 - q]);
-        }
-    } else {
-        for (int c = s0; c < s1; c += 4) {
-            Mat2 Tc[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (c + q < s1) Tc[q] = load_T(lp, item0 + c + q, lane);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (c + q < s1) u_step(c + q, Tc[q]);
-        }
-        for (int c = s1 - 1; c >= s0; c -= 4) {
-            Mat2 Tc[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (c - q >= s0) Tc[q] = load_T(lp, item0 + c - q, lane);
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (c - q >= s0) v_step(c - q, Tc[q]);
-        }
-    }
-    if (w == W - 1 && valid) {
-        // this wave's u now sits at the end of the chain: (u_N, u_{N-1})
-        const double2 Dm = csub(cscale(tau, ux), cmul(SR, uy));  // D = Dm 2^ue
-        const int eN = ue;
+    if (dir == 0 && r == 0 && valid) {
+        // the u walk ended at the end of the chain: (u_N, u_{N-1})
+        double2 uN = make_double2(hand[0][lane], hand[1][lane]);
+        double2 uNm1 = make_double2(hand[2][lane], hand[3][lane]);
+        int eN = (int)hand[4][lane];
+        renorm(uN, uNm1, eN);
+        const double2 Dm = csub(cscale(tau, uN), cmul(SR, uNm1));  // D = Dm 2^eN
         double* Dp = lp.Dd + (long long)chain * 3;
         Dp[0] = Dm.x;
         Dp[1] = Dm.y;
         Dp[2] = (double)eN;
         // transmission Gamma_L Gamma_R |G_{0,N-1}|^2, G_{0,N-1} = 1/D
         if (lp.kind == KIND_DOS && lp.chain_out && ch.out_slot >= 0) {
-            double iD2 = 1.0 / cabs2(Dm);
-            double gN = __builtin_amdgcn_ldexp(1.0, -eN);
-            double GamL = -2.0 * SL.y, GamR = -2.0 * SR.y;
+            const double iD2 = 1.0 / cabs2(Dm);
+            const double gN = __builtin_amdgcn_ldexp(1.0, -eN);
+            const double GamL = -2.0 * SL.y, GamR = -2.0 * SR.y;
             lp.chain_out[(long long)ch.out_slot * 2 + 1] = GamL * GamR * iD2 * gN * gN;
         }
     }
@@ -952,6 +883,8 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
     }
 
     // ---------------- pass 3 ----------------
+    double2 iW = make_double2(0.0, 0.0);  // KIND_EQ: 1 / mantissa of the segment Wronskian
+    int eW = 0;                           // and its exponent
     double dos_acc = 0.0;
     double2* Grow = WRITE_G ? (lp.diagG + (long long)ic.chain * N) : nullptr;
     for (int j = 0; j < nsub; ++j) {
@@ -961,6 +894,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
         int ce;
         ck.get(j, cx, cy, ce);
         double2 vr[K];
+        double2 v1;  // v_{p0+1}
         double Us[K];
 #pragma unroll
         for (int i = 0; i < K; ++i) Us[i] = (i < len) ? us[p0 - lo + i] : 0.0;
@@ -989,10 +923,20 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
                     }
                 }
             }
+            v1 = y;  // the last step (or the checkpoint itself when len == 1) left v_{p0+1} here
         }
         renorm(ux, uy, ue);
+        if (KIND == KIND_EQ && j == 0) {
+            // G_nn = u_n v_n / W with the Wronskian W = t_lo (u_{lo+1} v_lo - u_lo v_{lo+1})
+            // of this sweep's own u and v (see eq_full_body): W = Wm 2^(ue + ce)
+            double2 u1 = ux, u0 = uy;
+            fwd_step<0>(k, Us[0], u1, u0);
+            iW = crecip(cscale(ch.t_even, csub(cmul(u1, vr[0]), cmul(u0, v1))));
+            eW = ue + ce;
+        }
         const double sc_uv = __builtin_amdgcn_ldexp(1.0, ue + ce - eN);
-        const double2 C = cscale(sc_uv, cmul(coef, iD));
+        const double sc_w = __builtin_amdgcn_ldexp(1.0, ue + ce - eW);
+        const double2 C = KIND == KIND_EQ ? cscale(sc_w, cmul(coef, iW)) : cscale(sc_uv, cmul(coef, iD));
         const double sv = __builtin_amdgcn_ldexp(1.0, ce - eN);
         const double su = __builtin_amdgcn_ldexp(1.0, ue - eN);
         const double aL = GamL * sv * sv * iD2;
@@ -1001,7 +945,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
         const double kR = ch.cR * vmask * aR;
         SubCtx sc;
         sc.C = C;
-        sc.Gs = cscale(sc_uv, iD);
+        sc.Gs = KIND == KIND_EQ ? cscale(sc_w, iW) : cscale(sc_uv, iD);
         sc.kL = kL;
         sc.kR = kR;
         sc.aL = aL;
@@ -1030,13 +974,24 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 
 // Passes 2 and 3 of the equilibrium sweep over a full segment of K NS sites
 // staged in us: (x, y) is the right state (exponent e), (ux, uy) the left
-// state; ut = cD 2^(eoff + e_v) u with e_v the exponent of the first v
-// checkpoint (cD = coefficient / D, eoff = e_u - e_D and, for the scaled
-// recurrence, the exponent of the segment constant C).
+// state.  G_nn = u_n v_n / W with W the Wronskian
+//   W = t_lo (u_{lo+1} v_lo - u_lo v_{lo+1})
+// of the two solutions this sweep itself runs through the segment (u forward
+// from the left state, v backward from the right state), taken at the
+// segment's first link once v_lo and v_{lo+1} are known.  The exact
+// recurrences keep W constant along the chain, so this is the same G as
+// u_n v_n / D with D = t_{N-1} u_N - Sigma_R u_{N-1}; evaluated per segment
+// it cancels the normalisation error that the segment products of the scan
+// accumulate in u and v (which grows like the chain length times the
+// segment length near band edges, where the transfer matrices are nearly
+// defective), leaving only the error of the direct recurrences inside one
+// segment.  wp and wq give W's mantissa as wp u1 v0 - wq u0 v1 in the
+// recurrence's own variables (t_lo, t_lo for the plain recurrence; Im E and
+// t_lo^2 / Im E for the scaled one, where the segment constant C cancels).
 template <int K, int NS, class KC>
 __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, double2 ux, double2 uy, double2 x,
-                                             double2 y, int e, double2 cD, int eoff, double* red, int lane,
-                                             double* prow)
+                                             double2 y, int e, double2 coef, double wp, double wq, double* red,
+                                             int lane, double* prow)
 {
     // pass 2: v checkpoints at the right end of every sub-block
     double2 cx[NS], cy[NS];
@@ -1058,12 +1013,6 @@ __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, doub
     }
 
     // pass 3
-    {
-        const double sc = __builtin_amdgcn_ldexp(1.0, eoff + ce[0]);
-        const double2 C0 = cscale(sc, cD);
-        ux = cmul(C0, ux);
-        uy = cmul(C0, uy);
-    }
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         double2 vr[K];
@@ -1078,6 +1027,16 @@ __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, doub
                     bstep<0>(k, us[j * K + i], xx, yy);
                 vr[i - 1] = xx;
             }
+        }
+        if (j == 0) {
+            // ut = (coef / W) u, with W from (u_lo, u_{lo+1}) and (v_lo, v_{lo+1});
+            // the exponents of u and v cancel between u v and W
+            double2 u1 = ux, u0 = uy;
+            fstep<0>(k, us[0], u1, u0);  // u1 = u_{lo+1}, u0 = u_lo
+            const double2 Wm = csub(cscale(wp, cmul(u1, vr[0])), cscale(wq, cmul(u0, vr[1])));
+            const double2 C0 = cmul(coef, crecip(Wm));
+            ux = cmul(C0, ux);
+            uy = cmul(C0, uy);
         }
 #pragma unroll
         for (int i = 0; i < K; ++i) {
@@ -1099,15 +1058,15 @@ __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, doub
 
 // Equilibrium sweep of a full segment of NS sub-blocks (the common case),
 // with the chain coefficient folded into the left state: with
-//   ut = (coef / D) 2^(e_u + e_v - e_D) u
+//   ut = (coef / W) u
 // (a solution of the same linear recurrence) the contribution of site n is
 // Re(ut_n v_n), two instructions.  Between sub-blocks ut only changes by the
 // exact power of two of the next v checkpoint, so u needs no renormalisation:
 // |ut_n v_n| = |coef G_nn| stays bounded.  Checkpoint indices are static.
 // Waves whose chains all have t / |Im E| <= 2^24 run the scaled recurrence:
 // the entry states become w_lo = u_lo, w_{lo-1} = ai_{lo-1} u_{lo-1},
-// w'_{hi-1} = v_{hi-1}, w'_hi = ai_{hi-1} v_hi, and the segment constant
-// C = ai_lo ... ai_{hi-2} joins the weight (u_n v_n = C w_n w'_n).
+// w'_{hi-1} = v_{hi-1}, w'_hi = ai_{hi-1} v_hi (u_n v_n = C w_n w'_n with the
+// segment constant C, which also divides W and so drops out).
 template <int K, int NS>
 __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLaunch lp)
 {
@@ -1124,33 +1083,27 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     if (tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
     const double2 E = make_double2(ch.Er, ch.Ei);
-    const int N = tube.N;
     const int lo = ic.s * tube.L;  // the segment is [lo, lo + K NS), lo even
     double* us = us_all[wv];
     stage_segment(us, tube.U, lo, lo + K * NS, lane);
     const double* S = lp.St + (long long)item * 10 * 64 + lane;
     double2 ux = make_double2(S[0], S[64]), uy = make_double2(S[128], S[192]);
-    const int ue = (int)S[256];
     double2 x = make_double2(S[320], S[384]), y = make_double2(S[448], S[512]);
     const int e = (int)S[576];
-    const double* Dp = lp.Dd + (long long)ic.chain * 3;
-    const double2 iD = crecip(make_double2(Dp[0], Dp[1]));
-    const int eN = (int)Dp[2];
     const double vmask = ic.valid ? 1.0 : 0.0;
-    const double2 cD = cmul(make_double2(ch.cr * vmask, ch.ci * vmask), iD);
-    double* prow = lp.part + tube.part_off + (long long)ic.g * N + lo;
+    const double2 coef = make_double2(ch.cr * vmask, ch.ci * vmask);
+    double* prow = lp.part + tube.part_off + (long long)ic.g * tube.N + lo;
+    const double te = ch.t_even;  // t_lo: lo is even
 
     if (__all(scaled_ok(E, ch.t_even, tube.t_odd))) {
         const SCoefs ks = make_scoefs(E, ch.t_even, tube.t_odd);
         uy = cscale(ks.ai[1], uy);  // site lo - 1 is odd
         y = cscale(ks.ai[1], y);    // site hi - 1 is odd
-        double mC;
-        int eC;
-        scale_prod(ks, 0, K * NS - 1, mC, eC);
-        eq_full_body<K, NS>(ks, us, ux, uy, x, y, e, cscale(mC, cD), ue - eN + eC, red, lane, prow);
+        // W / C = t_lo (ai_lo w_{lo+1} w'_lo - w_lo w'_{lo+1} / ai_lo)
+        eq_full_body<K, NS>(ks, us, ux, uy, x, y, e, coef, E.y, te * te * ks.ie, red, lane, prow);
     } else {
         const Coefs k = make_coefs(E, ch.t_even, tube.t_odd);
-        eq_full_body<K, NS>(k, us, ux, uy, x, y, e, cD, ue - eN, red, lane, prow);
+        eq_full_body<K, NS>(k, us, ux, uy, x, y, e, coef, te, te, red, lane, prow);
     }
 }
 

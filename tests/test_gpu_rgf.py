@@ -324,24 +324,100 @@ def test_full_batched_workload_matches_oracle():
     assert rel_err(got, ref) < 1e-10
 
 
-@pytest.mark.parametrize("eta,flat", [(1e-5, True), (1e-5, False), (1e-8, True)])
+def bulk_dimer_G(E, te, to):
+    """Exact G_nn of the infinite two-hopping chain, the value every site of a
+    flat chain with matched leads has: E / sqrt((E^2 - (te+to)^2)(E^2 - (te-to)^2))
+    with Im G < 0, evaluated in 40-digit arithmetic at the double inputs."""
+    import mpmath as mp
+    mp.mp.dps = 40
+    out = []
+    for z in E:
+        zz = mp.mpc(z.real, z.imag)
+        a, b = mp.mpf(te) + mp.mpf(to), mp.mpf(to) - mp.mpf(te)
+        g = zz / mp.sqrt((zz - a) * (zz + a) * (zz - b) * (zz + b))
+        if mp.im(g) > 0:
+            g = -g
+        out.append(complex(g))
+    return np.array(out)
+
+
+def band_edge_energies(te, to, eta):
+    edges = [-(to + te), -(to - te), to - te, to + te]
+    offs = np.array([-30, -10, -3, -1, 0, 1, 3, 10, 30]) * eta
+    return np.array([e + o for e in edges for o in offs]) + 1j * eta
+
+
+@pytest.mark.parametrize("eta,flat", [(1e-5, True), (1e-5, False), (1e-8, True), (1e-8, False)])
 def test_band_edge_energies_match_oracle(eta, flat):
     """Real-axis energies within a few eta of the four band edges
     +-(gamma - beta), +-(gamma + beta) of the two-hopping chain, where the
     1-D density of states diverges and the recurrences barely decay; N = 4096.
-    eta = 1e-8 is the reference's default E_zPlus (NEGF_Common.H:386)."""
+    eta = 1e-8 is the reference's default E_zPlus (NEGF_Common.H:386).
+
+    On the flat chain (matched leads) every site has the closed-form bulk G,
+    so both this library and the oracle are measured against the exact value:
+    - eta = 1e-5: both within 2e-10 of it and of each other (the oracle, a
+      serial Riccati sweep, is at about 1e-11, this library at about 1e-10:
+      its segment transfer matrices are near defective there);
+    - eta = 1e-8: the problem's own condition number near the edges (about
+      1/eta) puts even the oracle 3e-8 from the exact value; this library
+      must stay within 10x the oracle's own error.
+    With a potential (smooth, +-0.05 eV) the same energies are off the local
+    band edges on most of the chain and the 1e-10 parity with the oracle
+    holds.  Both the general sweep (diag G written) and the full-segment
+    sweep (charge only) are checked."""
     N = 4096
     te, to = BETA17, GAMMA
     U = np.zeros(N) if flat else smooth_potential(N, seed=9, lo=-0.05, hi=0.05)
-    edges = [-(to + te), -(to - te), to - te, to + te]
-    offs = np.array([-30, -10, -3, -1, 0, 1, 3, 10, 30]) * eta
-    E = np.array([e + o for e in edges for o in offs]) + 1j * eta
+    E = band_edge_energies(te, to, eta)
     rng = np.random.default_rng(4)
     coef = rng.normal(size=E.size) + 1j * rng.normal(size=E.size)
     Uc = (U[0], U[-1])
     acc_ref, G_ref, st = O.accumulate_diagG(U, te, to, Uc, te, to, E, coef, want_diag=True)
     acc, G = run_diag(U, te, to, Uc, E, coef)
-    assert np.all(np.isfinite(acc))
-    assert rel_err(acc, acc_ref) < 1e-10
-    big = np.abs(G_ref) > 1e-6 * np.abs(G_ref).max()
-    assert (np.abs(G - G_ref)[big] / np.abs(G_ref)[big]).max() < 1e-9
+    acc_f, _ = run_diag(U, te, to, Uc, E, coef, want_G=False)
+    assert np.all(np.isfinite(acc)) and np.all(np.isfinite(acc_f))
+    if not flat:
+        assert rel_err(acc, acc_ref) < 1e-10
+        assert rel_err(acc_f, acc_ref) < 1e-10
+        return
+    Gx = bulk_dimer_G(E, te, to)
+    acc_x = np.full(N, np.sum((coef * Gx).real))
+    g_err = (np.abs(G - Gx[:, None]) / np.abs(Gx[:, None])).max()
+    g_err_ref = (np.abs(G_ref - Gx[:, None]) / np.abs(Gx[:, None])).max()
+    errs = dict(gpu=rel_err(acc, acc_x), gpu_full=rel_err(acc_f, acc_x), oracle=rel_err(acc_ref, acc_x))
+    print(f"eta={eta:g}: charge vs exact {errs}; G vs exact gpu {g_err:.2e} oracle {g_err_ref:.2e}")
+    if eta >= 1e-5:
+        assert max(errs.values()) < 2e-10
+        assert rel_err(acc, acc_ref) < 2e-10 and rel_err(acc_f, acc_ref) < 2e-10
+        assert g_err < 2e-10
+    else:
+        assert errs["gpu"] < 10 * errs["oracle"] and errs["gpu_full"] < 10 * errs["oracle"]
+        assert g_err < 10 * g_err_ref
+
+
+@pytest.mark.parametrize("eta,flat", [(1e-5, True), (1e-5, False), (1e-8, False)])
+def test_band_edge_noneq_matches_oracle(eta, flat):
+    """Non-equilibrium (contact-resolved) accumulation at the band-edge
+    energies against the oracle's O(N^2) contact walk, N = 4096.  |G_n0|^2 and
+    |G_n,N-1|^2 carry the normalisation of the states along the whole chain,
+    which the segment products of the scan hold to about 2e-10 next to the
+    outer band edges of the flat chain (eta = 1e-5); with a potential the
+    1e-10 parity holds."""
+    lib = _lib.load()
+    N = 4096
+    te, to = BETA17, GAMMA
+    U = np.zeros(N) if flat else smooth_potential(N, seed=9, lo=-0.05, hi=0.05)
+    E = band_edge_energies(te, to, eta)
+    rng = np.random.default_rng(5)
+    cL, cR = rng.normal(size=E.size), rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    ref, st = O.accumulate_noneq(U, te, to, Uc, te, to, E, cL, cR)
+    p = chain_params(N, te, to, Uc)
+    dU, dE, dL, dR = dev_array(U), dev_array(E), dev_array(cL), dev_array(cR)
+    acc = dev_zeros(N)
+    _lib.check(lib.negf_rgf_noneq_accumulate(p, ptr(dU), E.size, ptr(dE), ptr(dL), ptr(dR),
+                                             None, None, ptr(acc), None, None))
+    err = rel_err(to_np(acc), ref)
+    print(f"eta={eta:g} flat={flat}: noneq vs oracle {err:.2e}")
+    assert err < (3e-10 if flat else 1e-10)
