@@ -83,6 +83,8 @@ SIGNATURES = {
     "negf_version": (C.c_int, []),
     "negf_max_sites": (C.c_int, []),
     "negf_rgf_diag_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp]),
+    "negf_rgf_diag_accumulate_noalloc": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp,
+                                                   vp]),
     "negf_rgf_noneq_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp, vp, vp, vp]),
     "negf_rgf_dos": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp]),
     "negf_rgf_plan_create": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, C.c_int, C.POINTER(vp)]),

@@ -126,10 +126,10 @@ static void launch_sweep_eq_full(int ns, int nblk, const WLaunch& lp, hipStream_
 // ---------------------------------------------------------------- plan
 RgfPlan::~RgfPlan()
 {
-    DevBuf* bufs[] = {&d_item_list_,   &d_tubes_,       &d_item_tube_,   &d_chain_tube_,   &d_sblk_tube_,    &d_sblk_chain_,
-                      &d_Tm_,          &d_St_,          &d_Dd_,           &d_part_,         &d_tmp_,
-                      &d_dos_part_,    &d_status_,      &d_red_src_off_,  &d_red_nrows_,    &d_red_N_,
-                      &d_red_dst_off_, &d_red2_src_off_, &d_red2_nrows_,  &d_red2_dst_off_};
+    DevBuf* bufs[] = {&d_item_list_, &d_tubes_,    &d_item_tube_, &d_chain_tube_,  &d_sblk_tube_,
+                      &d_sblk_chain_, &d_Tm_,       &d_St_,        &d_Dd_,          &d_part_,
+                      &d_dos_part_,   &d_status_,   &d_red_src_off_, &d_red_nrows_, &d_red_N_,
+                      &d_red_dst_off_};
     for (DevBuf* b : bufs) b->release();
     for (hipEvent_t e : ev_start_) (void)hipEventDestroy(e);
     for (hipEvent_t e : ev_stop_) (void)hipEventDestroy(e);
@@ -148,6 +148,30 @@ int RgfPlan::kernel_time(double* total_ms, int* launches)
     *launches = (int)ev_used_;
     ev_used_ = 0;
     return NEGF_OK;
+}
+
+// Kernel-side descriptor of one tube whose items start at item0 and whose
+// partial rows start at part_off.
+static WTube make_wtube(const RgfTube& tb, int P, int L, int item0, long long part_off)
+{
+    WTube w;
+    std::memset(&w, 0, sizeof(w));
+    w.N = tb.N;
+    w.L = L;
+    w.P = P;
+    w.nsub = L / NEGF_SUB_K;
+    w.t_odd = tb.t_odd;
+    w.Uc0 = tb.Uc0;
+    w.Uc1 = tb.Uc1;
+    w.U = tb.dU;
+    w.Ucp = tb.dUc;
+    w.chain0 = tb.chain0;
+    w.nchain = tb.nchain;
+    w.item0 = item0;
+    w.part_off = part_off;
+    for (int q = 0; q < 3; ++q) w.sample[q] = tb.sample[q];
+    w.ext = tb.ext;
+    return w;
 }
 
 template <class T>
@@ -180,9 +204,9 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     std::vector<WTube> wt(ntubes_);
     std::vector<int> item_tube, chain_tube(nchains, 0), sblk_tube, sblk_chain;
     std::vector<int> fast_items[NEGF_SUB_NS + 1], slow_items;
-    std::vector<long long> rsrc, rdst, tdst, t2src;
-    std::vector<int> rnrows, rN, t2rows;
-    long long poff = 0, toff = 0;
+    std::vector<long long> rsrc, rdst;
+    std::vector<int> rnrows, rN;
+    long long poff = 0;
     int items = 0;
     max_N_ = 0;
     max_P_ = 0;
@@ -201,22 +225,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
             return NEGF_ENOTSUP;
         }
         WTube& w = wt[t];
-        std::memset(&w, 0, sizeof(w));
-        w.N = tb.N;
-        w.L = L;
-        w.P = P;
-        w.nsub = L / NEGF_SUB_K;
-        w.t_odd = tb.t_odd;
-        w.Uc0 = tb.Uc0;
-        w.Uc1 = tb.Uc1;
-        w.U = tb.dU;
-        w.Ucp = tb.dUc;
-        w.chain0 = tb.chain0;
-        w.nchain = tb.nchain;
-        w.item0 = items;
-        w.part_off = poff;
-        for (int q = 0; q < 3; ++q) w.sample[q] = tb.sample[q];
-        w.ext = tb.ext;
+        w = make_wtube(tb, P, L, items, poff);
         const int G = (tb.nchain + 63) / 64;
         for (int i = 0; i < G * P; ++i) {
             item_tube.push_back(t);
@@ -240,11 +249,6 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         rnrows.push_back(G);
         rN.push_back(tb.N);
         poff += (long long)G * tb.N;
-        int nch = (G + reduce_chunk_ - 1) / reduce_chunk_;
-        tdst.push_back(toff);
-        t2src.push_back(toff);
-        t2rows.push_back(nch);
-        toff += (long long)nch * tb.N;
         rdst.push_back(tb.out_off);
         max_N_ = std::max(max_N_, tb.N);
         max_P_ = std::max(max_P_, P);
@@ -252,13 +256,11 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     }
     nitems_ = items;
     nsblk_ = (int)sblk_tube.size();
-    max_chunks_ = (max_groups_ + reduce_chunk_ - 1) / reduce_chunk_;
     int rc;
     if ((rc = d_Tm_.ensure((size_t)items * 9 * 64 * sizeof(double)))) return rc;
     if ((rc = d_St_.ensure((size_t)items * 10 * 64 * sizeof(double)))) return rc;
     if ((rc = d_Dd_.ensure((size_t)nchains * 3 * sizeof(double)))) return rc;
     if ((rc = d_part_.ensure(std::max<long long>(1, poff) * sizeof(double)))) return rc;
-    if ((rc = d_tmp_.ensure(std::max<long long>(1, toff) * sizeof(double)))) return rc;
     if (kind == KIND_DOS && (rc = d_dos_part_.ensure((size_t)items * 64 * sizeof(double)))) return rc;
     if ((rc = upload(d_tubes_, wt))) return rc;
     if ((rc = upload(d_item_tube_, item_tube))) return rc;
@@ -280,12 +282,112 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     if ((rc = upload(d_red_src_off_, rsrc))) return rc;
     if ((rc = upload(d_red_nrows_, rnrows))) return rc;
     if ((rc = upload(d_red_N_, rN))) return rc;
-    if ((rc = upload(d_red_dst_off_, tdst))) return rc;
-    if ((rc = upload(d_red2_src_off_, t2src))) return rc;
-    if ((rc = upload(d_red2_nrows_, t2rows))) return rc;
-    if ((rc = upload(d_red2_dst_off_, rdst))) return rc;
+    if ((rc = upload(d_red_dst_off_, rdst))) return rc;
     if ((rc = d_status_.ensure(sizeof(int)))) return rc;
     NEGF_HIP_TRY(hipMemset(d_status_.p, 0, sizeof(int)));
+    return NEGF_OK;
+}
+
+// Everything one launch sequence reads: device tables and workspace, and the
+// host counts that size the launches.  A plan fills it from its own buffers;
+// the no-allocation entry point from the fixed scratch array.
+struct RgfLayout {
+    int kind = 0;
+    bool write_G = false;
+    int nitems = 0, nchains = 0, ntubes = 0, nsblk = 0, max_N = 0;
+    ChainDesc* chains = nullptr;
+    const WTube* tubes = nullptr;
+    const int* item_tube = nullptr;
+    const int* chain_tube = nullptr;
+    const int* sblk_tube = nullptr;
+    const int* sblk_chain = nullptr;
+    const int* item_list = nullptr;  // fast items by sub-block count, then the general-sweep items
+    int fast_off[NEGF_SUB_NS + 1] = {}, fast_n[NEGF_SUB_NS + 1] = {};
+    int slow_off = 0, slow_n = 0;
+    double *Tm = nullptr, *St = nullptr, *Dd = nullptr, *part = nullptr, *dos_part = nullptr;
+    int* status = nullptr;
+    const long long* red_src_off = nullptr;  // fixed-order row reduction tables
+    const int* red_nrows = nullptr;
+    const int* red_N = nullptr;
+    const long long* red_dst_off = nullptr;
+    DecParams dec;
+};
+
+static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_chain_out, const int* tube_active)
+{
+    WLaunch lp;
+    std::memset(&lp, 0, sizeof(lp));
+    lp.chains = y.chains;
+    lp.tubes = y.tubes;
+    lp.item_tube = y.item_tube;
+    lp.nitems = y.nitems;
+    lp.Tm = y.Tm;
+    lp.St = y.St;
+    lp.Dd = y.Dd;
+    lp.part = y.part;
+    lp.diagG = d_diagG;
+    lp.chain_out = d_chain_out;
+    lp.dos_part = y.dos_part;
+    lp.sblk_tube = y.sblk_tube;
+    lp.sblk_chain = y.sblk_chain;
+    lp.status = y.status;
+    lp.kind = y.kind;
+    lp.dec_use = y.dec.use;
+    lp.dec_layers = y.dec.layers;
+    lp.dec_max_iter = y.dec.max_iter;
+    lp.dec_rel_error = y.dec.rel_error;
+    lp.tube_active = tube_active;
+    return lp;
+}
+
+// Transfer, scan and sweeps of one layout on a stream.
+static int launch_rgf_kernels(const RgfLayout& y, const WLaunch& lp, hipStream_t st)
+{
+    hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((y.nitems + 3) / 4), dim3(256), 0, st, lp);
+    hipLaunchKernelGGL(scan_kernel, dim3(2 * y.nsblk), dim3(64 * SCAN_WAVES), 0, st, lp, y.nsblk);
+    for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
+        if (y.fast_n[ns] == 0) continue;
+        WLaunch lf = lp;
+        lf.item_list = y.item_list + y.fast_off[ns];
+        lf.nlist = y.fast_n[ns];
+        if (y.kind == KIND_EQ)
+            launch_sweep_eq_full(ns, (y.fast_n[ns] + 3) / 4, lf, st);
+        else
+            launch_sweep_neq_full(ns, (y.fast_n[ns] + 3) / 4, lf, st);
+    }
+    WLaunch ls = lp;
+    ls.item_list = y.item_list + y.slow_off;
+    ls.nlist = y.slow_n;
+    const int nblk_slow = (y.slow_n + 3) / 4;
+    if (y.slow_n == 0) {
+        // nothing left for the general sweep
+    } else if (y.kind == KIND_EQ) {
+        if (y.write_G)
+            launch_sweep<KIND_EQ, true>(nblk_slow, ls, st);
+        else
+            launch_sweep<KIND_EQ, false>(nblk_slow, ls, st);
+    } else if (y.kind == KIND_NONEQ) {
+        launch_sweep<KIND_NONEQ, false>(nblk_slow, ls, st);
+    } else {
+        launch_sweep<KIND_DOS, false>(nblk_slow, ls, st);
+    }
+    NEGF_HIP_TRY(hipGetLastError());
+    return NEGF_OK;
+}
+
+// Per-chain DOS traces, or the fixed-order reduction of the group rows.
+static int launch_rgf_finish(const RgfLayout& y, const WLaunch& lp, double* d_out, hipStream_t st,
+                             const int* tube_active)
+{
+    if (y.kind == KIND_DOS) {
+        hipLaunchKernelGGL(dos_finish_kernel, dim3((y.nchains + 255) / 256), dim3(256), 0, st, lp, y.ntubes);
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
+    dim3 g1((y.max_N + 15) / 16, y.ntubes);
+    hipLaunchKernelGGL(reduce_rows1_kernel, g1, dim3(256), 0, st, y.part, y.red_src_off, y.red_nrows, y.red_N,
+                       d_out, y.red_dst_off, y.tubes, tube_active);
+    NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }
 
@@ -307,32 +409,43 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         set_error("output buffer missing");
         return NEGF_EARG;
     }
-    WLaunch lp;
-    std::memset(&lp, 0, sizeof(lp));
-    lp.chains = d_chains_;
-    lp.tubes = d_tubes_.as<WTube>();
-    lp.item_tube = d_item_tube_.as<int>();
-    lp.nitems = nitems_;
-    lp.Tm = d_Tm_.as<double>();
-    lp.St = d_St_.as<double>();
-    lp.Dd = d_Dd_.as<double>();
-    lp.part = d_part_.as<double>();
-    lp.diagG = d_diagG;
-    lp.chain_out = d_chain_out;
-    lp.dos_part = d_dos_part_.as<double>();
-    lp.sblk_tube = d_sblk_tube_.as<int>();
-    lp.sblk_chain = d_sblk_chain_.as<int>();
-    lp.status = d_status_.as<int>();
-    lp.kind = kind_;
-    lp.dec_use = dec_.use;
-    lp.dec_layers = dec_.layers;
-    lp.dec_max_iter = dec_.max_iter;
-    lp.dec_rel_error = dec_.rel_error;
-    lp.tube_active = tube_active;
+    RgfLayout y;
+    y.kind = kind_;
+    y.write_G = write_G_;
+    y.nitems = nitems_;
+    y.nchains = nchains_;
+    y.ntubes = ntubes_;
+    y.nsblk = nsblk_;
+    y.max_N = max_N_;
+    y.chains = d_chains_;
+    y.tubes = d_tubes_.as<WTube>();
+    y.item_tube = d_item_tube_.as<int>();
+    y.chain_tube = d_chain_tube_.as<int>();
+    y.sblk_tube = d_sblk_tube_.as<int>();
+    y.sblk_chain = d_sblk_chain_.as<int>();
+    y.item_list = d_item_list_.as<int>();
+    for (int ns = 0; ns <= NEGF_SUB_NS; ++ns) {
+        y.fast_off[ns] = fast_off_[ns];
+        y.fast_n[ns] = fast_n_[ns];
+    }
+    y.slow_off = slow_off_;
+    y.slow_n = slow_n_;
+    y.Tm = d_Tm_.as<double>();
+    y.St = d_St_.as<double>();
+    y.Dd = d_Dd_.as<double>();
+    y.part = d_part_.as<double>();
+    y.dos_part = d_dos_part_.as<double>();
+    y.status = d_status_.as<int>();
+    y.red_src_off = d_red_src_off_.as<long long>();
+    y.red_nrows = d_red_nrows_.as<int>();
+    y.red_N = d_red_N_.as<int>();
+    y.red_dst_off = d_red_dst_off_.as<long long>();
+    y.dec = dec_;
+    const WLaunch lp = launch_params(y, d_diagG, d_chain_out, tube_active);
 
     if (dec_.use) {
-        hipLaunchKernelGGL(sigma_dec_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, lp,
-                           d_chain_tube_.as<int>(), nchains_);
+        hipLaunchKernelGGL(sigma_dec_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, lp, y.chain_tube,
+                           nchains_);
         NEGF_HIP_TRY(hipGetLastError());
     }
     if (timing_) {
@@ -345,50 +458,13 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         }
         NEGF_HIP_TRY(hipEventRecord(ev_start_[ev_used_], st));
     }
-    hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((nitems_ + 3) / 4), dim3(256), 0, st, lp);
-    hipLaunchKernelGGL(scan_kernel, dim3(2 * nsblk_), dim3(64 * SCAN_WAVES), 0, st, lp, nsblk_);
-    for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
-        if (fast_n_[ns] == 0) continue;
-        WLaunch lf = lp;
-        lf.item_list = d_item_list_.as<int>() + fast_off_[ns];
-        lf.nlist = fast_n_[ns];
-        if (kind_ == KIND_EQ)
-            launch_sweep_eq_full(ns, (fast_n_[ns] + 3) / 4, lf, st);
-        else
-            launch_sweep_neq_full(ns, (fast_n_[ns] + 3) / 4, lf, st);
-    }
-    lp.item_list = d_item_list_.as<int>() + slow_off_;
-    lp.nlist = slow_n_;
-    const int nblk_slow = (slow_n_ + 3) / 4;
-    if (slow_n_ == 0) {
-        // nothing left for the general sweep
-    } else if (kind_ == KIND_EQ) {
-        if (write_G_)
-            launch_sweep<KIND_EQ, true>(nblk_slow, lp, st);
-        else
-            launch_sweep<KIND_EQ, false>(nblk_slow, lp, st);
-    } else if (kind_ == KIND_NONEQ) {
-        launch_sweep<KIND_NONEQ, false>(nblk_slow, lp, st);
-    } else {
-        launch_sweep<KIND_DOS, false>(nblk_slow, lp, st);
-    }
-    NEGF_HIP_TRY(hipGetLastError());
+    int rc = launch_rgf_kernels(y, lp, st);
+    if (rc) return rc;
     if (timing_) {
         NEGF_HIP_TRY(hipEventRecord(ev_stop_[ev_used_], st));
         ++ev_used_;
     }
-    if (kind_ == KIND_DOS) {
-        hipLaunchKernelGGL(dos_finish_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, lp, ntubes_);
-        NEGF_HIP_TRY(hipGetLastError());
-        return NEGF_OK;
-    }
-    // fixed-order reduction of the group rows
-    dim3 g1((max_N_ + 15) / 16, ntubes_);
-    hipLaunchKernelGGL(reduce_rows1_kernel, g1, dim3(256), 0, st, d_part_.as<double>(),
-                       d_red_src_off_.as<long long>(), d_red_nrows_.as<int>(), d_red_N_.as<int>(), d_out,
-                       d_red2_dst_off_.as<long long>(), d_tubes_.as<WTube>(), tube_active);
-    NEGF_HIP_TRY(hipGetLastError());
-    return NEGF_OK;
+    return launch_rgf_finish(y, lp, d_out, st, tube_active);
 }
 
 int RgfPlan::check_status(hipStream_t st, int* status)
@@ -514,6 +590,75 @@ static int finish_status(RgfPlan& plan, hipStream_t st, bool report_root = false
     return NEGF_OK;
 }
 
+// ---------------------------------------------------------------- fixed scratch
+// Workspace of negf_rgf_diag_accumulate_noalloc: a file-scope device array of
+// NEGF_SCRATCH_BYTES that every GPU the library runs on holds from module load
+// to process end.  In host code the array's name is only a host-side handle:
+// the device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current, and only that address is passed
+// to kernels.
+#define NEGF_SCRATCH_BYTES (64ull << 20)
+#define NEGF_SCRATCH_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[NEGF_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static void* g_scratch_addr[NEGF_SCRATCH_MAX_DEVICES];
+
+static int scratch_address(void** out)
+{
+    int dev = 0;
+    NEGF_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NEGF_SCRATCH_MAX_DEVICES) {
+        set_error("device %d outside the %d devices the fixed scratch tracks", dev, NEGF_SCRATCH_MAX_DEVICES);
+        return NEGF_ENOTSUP;
+    }
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", dev, hipGetErrorString(e));
+            return NEGF_EDEVICE;
+        }
+        g_scratch_addr[dev] = ptr;
+    }
+    *out = g_scratch_addr[dev];
+    return NEGF_OK;
+}
+
+// Tables of a one-tube launch, written on the device: item -> tube, the item
+// list (full segments first when the unrolled sweep takes them, in the order
+// RgfPlan::build gives), chain groups, the row-reduction tables, the tube
+// descriptor, and the status word cleared.
+__global__ void one_tube_tables_kernel(WTube tb, WTube* tubes, int items, int P, int nfull, int fast, int* item_tube,
+                                       int* item_list, int G, int* sblk_tube, int* sblk_chain, int nE,
+                                       int* chain_tube, long long* red_src_off, int* red_nrows, int* red_N,
+                                       long long* red_dst_off, int* status)
+{
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    const int nf = fast ? G * nfull : 0;
+    for (int i = i0; i < items; i += stride) {
+        item_tube[i] = 0;
+        int item = i;
+        if (fast) item = i < nf ? (i / nfull) * P + i % nfull : (i - nf) * P + (P - 1);
+        item_list[i] = item;
+    }
+    for (int g = i0; g < G; g += stride) {
+        sblk_tube[g] = 0;
+        sblk_chain[g] = g;
+    }
+    for (int c = i0; c < nE; c += stride) chain_tube[c] = 0;
+    if (i0 == 0) {
+        *tubes = tb;
+        red_src_off[0] = 0;
+        red_nrows[0] = G;
+        red_N[0] = tb.N;
+        red_dst_off[0] = 0;
+        *status = 0;
+    }
+}
+
 }  // namespace negf
 
 using namespace negf;
@@ -545,6 +690,115 @@ int negf_rgf_diag_accumulate(const negf_chain_params* p, const double* d_U, int 
     if ((rc = c.plan.build(KIND_EQ, d_diagG != nullptr, tubes, c.chains.as<ChainDesc>(), nE, dec_of(p)))) return rc;
     if ((rc = c.plan.enqueue(d_acc, (double2*)d_diagG, nullptr, st))) return rc;
     return finish_status(c.plan, st);
+}
+
+int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d_U, int nE,
+                                     const negf_complex* d_E, const negf_complex* d_coef, double* d_acc,
+                                     negf_complex* d_diagG, void* stream)
+{
+    clear_error();
+    int rc = check_params(p, d_U, nE, d_E);
+    if (rc) return rc;
+    if (!d_acc) {
+        set_error("d_acc is NULL");
+        return NEGF_EARG;
+    }
+    if (p->use_decimation && (p->decimation_layers < 2 || p->decimation_layers > NEGF_MAX_DECIMATION_LAYERS)) {
+        set_error("decimation_layers must be in [2, %d]", NEGF_MAX_DECIMATION_LAYERS);
+        return NEGF_EARG;
+    }
+    int P, L;
+    if (!choose_segments(p->N, p->segment_length, &P, &L)) {
+        set_error("chain of %d sites exceeds the supported maximum %d", p->N, max_sites());
+        return NEGF_ENOTSUP;
+    }
+    const size_t N = (size_t)p->N, E = (size_t)nE;
+    const size_t G = (E + 63) / 64, items = G * (size_t)P;
+    size_t off = 0;
+    auto take = [&off](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_ch = take(E * sizeof(ChainDesc)), o_tb = take(sizeof(WTube));
+    const size_t o_it = take(items * sizeof(int)), o_il = take(items * sizeof(int));
+    const size_t o_ct = take(E * sizeof(int)), o_gt = take(G * sizeof(int)), o_gc = take(G * sizeof(int));
+    const size_t o_Tm = take(items * 9 * 64 * sizeof(double)), o_St = take(items * 10 * 64 * sizeof(double));
+    const size_t o_Dd = take(E * 3 * sizeof(double)), o_pt = take(G * N * sizeof(double));
+    const size_t o_r1 = take(sizeof(long long)), o_r2 = take(sizeof(int)), o_r3 = take(sizeof(int));
+    const size_t o_r4 = take(sizeof(long long)), o_stat = take(sizeof(int));
+    if (off > NEGF_SCRATCH_BYTES) {
+        set_error("%d energies x %d sites need %zu bytes of workspace, more than the %llu-byte fixed scratch of "
+                  "negf_rgf_diag_accumulate_noalloc (use negf_rgf_diag_accumulate)",
+                  nE, p->N, off, (unsigned long long)NEGF_SCRATCH_BYTES);
+        return NEGF_EARG;
+    }
+    void* base = nullptr;
+    if ((rc = scratch_address(&base))) return rc;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    hipStream_t st = (hipStream_t)stream;
+
+    RgfLayout y;
+    y.kind = KIND_EQ;
+    y.write_G = d_diagG != nullptr;
+    y.nitems = (int)items;
+    y.nchains = nE;
+    y.ntubes = 1;
+    y.nsblk = (int)G;
+    y.max_N = p->N;
+    y.chains = reinterpret_cast<ChainDesc*>(b + o_ch);
+    WTube* d_tubes = reinterpret_cast<WTube*>(b + o_tb);
+    y.tubes = d_tubes;
+    y.item_tube = reinterpret_cast<int*>(b + o_it);
+    y.item_list = reinterpret_cast<int*>(b + o_il);
+    y.chain_tube = reinterpret_cast<int*>(b + o_ct);
+    y.sblk_tube = reinterpret_cast<int*>(b + o_gt);
+    y.sblk_chain = reinterpret_cast<int*>(b + o_gc);
+    y.Tm = reinterpret_cast<double*>(b + o_Tm);
+    y.St = reinterpret_cast<double*>(b + o_St);
+    y.Dd = reinterpret_cast<double*>(b + o_Dd);
+    y.part = reinterpret_cast<double*>(b + o_pt);
+    y.red_src_off = reinterpret_cast<long long*>(b + o_r1);
+    y.red_nrows = reinterpret_cast<int*>(b + o_r2);
+    y.red_N = reinterpret_cast<int*>(b + o_r3);
+    y.red_dst_off = reinterpret_cast<long long*>(b + o_r4);
+    y.status = reinterpret_cast<int*>(b + o_stat);
+    y.dec = dec_of(p);
+    // the same split as RgfPlan::build: full segments to the unrolled sweep
+    // unless diag G is written
+    const int nfull = (p->N % L == 0) ? P : P - 1;
+    const bool fast = !y.write_G;
+    if (fast) {
+        y.fast_off[L / NEGF_SUB_K] = 0;
+        y.fast_n[L / NEGF_SUB_K] = (int)G * nfull;
+        y.slow_off = (int)G * nfull;
+        y.slow_n = (int)items - (int)G * nfull;
+    } else {
+        y.slow_off = 0;
+        y.slow_n = (int)items;
+    }
+    const WTube tb = make_wtube(one_tube(p, d_U, nE), P, L, 0, 0);
+
+    hipLaunchKernelGGL(pack_eq_chains, dim3((nE + 255) / 256), dim3(256), 0, st, nE, (const double2*)d_E,
+                       (const double2*)d_coef, p->t_even, y.chains);
+    const int nt = (int)std::min<size_t>(1024, (std::max(items, E) + 255) / 256);
+    hipLaunchKernelGGL(one_tube_tables_kernel, dim3(nt), dim3(256), 0, st, tb, d_tubes, (int)items, P, nfull,
+                       fast ? 1 : 0, const_cast<int*>(y.item_tube), const_cast<int*>(y.item_list), (int)G,
+                       const_cast<int*>(y.sblk_tube), const_cast<int*>(y.sblk_chain), nE,
+                       const_cast<int*>(y.chain_tube), const_cast<long long*>(y.red_src_off),
+                       const_cast<int*>(y.red_nrows), const_cast<int*>(y.red_N),
+                       const_cast<long long*>(y.red_dst_off), y.status);
+    NEGF_HIP_TRY(hipGetLastError());
+    const WLaunch lp = launch_params(y, (double2*)d_diagG, nullptr, nullptr);
+    if (y.dec.use) {
+        hipLaunchKernelGGL(sigma_dec_kernel, dim3((nE + 255) / 256), dim3(256), 0, st, lp, y.chain_tube, nE);
+        NEGF_HIP_TRY(hipGetLastError());
+    }
+    if ((rc = launch_rgf_kernels(y, lp, st))) return rc;
+    if ((rc = launch_rgf_finish(y, lp, d_acc, st, nullptr))) return rc;
+    // like negf_rgf_diag_accumulate: the results are complete on return
+    NEGF_HIP_TRY(hipStreamSynchronize(st));
+    return NEGF_OK;
 }
 
 int negf_rgf_noneq_accumulate(const negf_chain_params* p, const double* d_U, int nE, const negf_complex* d_E,

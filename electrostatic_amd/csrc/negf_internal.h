@@ -97,17 +97,15 @@ class RgfPlan {
     int kind_ = 0;
     bool write_G_ = false;
     int nitems_ = 0, nchains_ = 0, ntubes_ = 0, nsblk_ = 0;
-    int max_N_ = 0, max_P_ = 0, max_groups_ = 0, max_chunks_ = 0;
-    int reduce_chunk_ = 16;
+    int max_N_ = 0, max_P_ = 0, max_groups_ = 0;
     ChainDesc* d_chains_ = nullptr;
     DecParams dec_;
     std::vector<RgfTube> tubes_;
     DevBuf d_tubes_, d_item_tube_, d_chain_tube_, d_sblk_tube_, d_sblk_chain_, d_item_list_;
     int fast_off_[NEGF_SUB_NS + 1] = {}, fast_n_[NEGF_SUB_NS + 1] = {};  // unrolled sweep items by sub-blocks
     int slow_off_ = 0, slow_n_ = 0;                                      // general sweep items
-    DevBuf d_Tm_, d_St_, d_Dd_, d_part_, d_tmp_, d_dos_part_, d_status_;
-    DevBuf d_red_src_off_, d_red_nrows_, d_red_N_, d_red_dst_off_;   // pass 1
-    DevBuf d_red2_src_off_, d_red2_nrows_, d_red2_dst_off_;          // pass 2
+    DevBuf d_Tm_, d_St_, d_Dd_, d_part_, d_dos_part_, d_status_;
+    DevBuf d_red_src_off_, d_red_nrows_, d_red_N_, d_red_dst_off_;  // fixed-order row reduction
     bool timing_ = false;
     std::vector<hipEvent_t> ev_start_, ev_stop_;
     size_t ev_used_ = 0;

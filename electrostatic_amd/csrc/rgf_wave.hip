@@ -1290,29 +1290,6 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_neq_full_kernel(WL
     }
 }
 
-// Sum partial rows in a fixed order.  Row r of group t starts at
-// src + src_off[t] + r*N_t; output row c (rows [c*chunk, (c+1)*chunk)) goes to
-// dst + dst_off[t] + c*N_t.
-__global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restrict__ src,
-                                                          const long long* __restrict__ src_off,
-                                                          const int* __restrict__ nrows,
-                                                          const int* __restrict__ Ns, int chunk,
-                                                          double* __restrict__ dst,
-                                                          const long long* __restrict__ dst_off)
-{
-    const int t = blockIdx.z;
-    const int N = Ns[t];
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    const int c = blockIdx.y;
-    const int r_begin = c * chunk;
-    const int r_end = min(nrows[t], r_begin + chunk);
-    if (n >= N || r_begin >= r_end) return;
-    const double* p = src + src_off[t] + n;
-    double acc = 0.0;
-    for (int r = r_begin; r < r_end; ++r) acc += p[(long long)r * N];
-    dst[dst_off[t] + (long long)c * N + n] = acc;
-}
-
 // One-pass fixed-order sum of the group rows of every tube: a block covers
 // 16 sites x 16 row slices (slice k sums rows k, k+16, ...), the slices are
 // then added in slice order.

@@ -71,6 +71,25 @@ int negf_rgf_diag_accumulate(const negf_chain_params *p, const double *d_U,
                              const negf_complex *d_coef, double *d_acc,
                              negf_complex *d_diagG, void *stream);
 
+/* negf_rgf_diag_accumulate without any device allocation.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results.  The workspace (chain
+ * descriptors, segment matrices and states, partial rows, tables) lives in
+ * one fixed 64 MiB device array of the library, which every GPU the library
+ * runs on holds until the process ends; a problem whose workspace exceeds it
+ * (about 5e7 energy-sites at the default segment length, e.g. 8192 energies
+ * x 4096 sites needs 45 MB) fails with NEGF_EARG before anything is
+ * launched.  All calls on one GPU share that array: calls that overlap in
+ * time on one GPU (from other host threads or streams) race, and the caller
+ * must serialise them.  Synchronises the stream before returning. */
+int negf_rgf_diag_accumulate_noalloc(const negf_chain_params *p,
+                                     const double *d_U, int nE,
+                                     const negf_complex *d_E,
+                                     const negf_complex *d_coef,
+                                     double *d_acc, negf_complex *d_diagG,
+                                     void *stream);
+
 /* Non-equilibrium form (Compute_RhoNonEq kernel NEGF_Common.cpp:3377-3497):
  * acc[n] = sum_k coefL_k Gamma_L |G_n0|^2 + coefR_k Gamma_R |G_n,N-1|^2.
  * d_integrand (optional, [nE][3]) receives the same integrand with
