@@ -58,6 +58,7 @@ class negf_tube_params(C.Structure):
         ("flag_adaptive_integration_limits", C.c_int),
         ("integrand_correction_interval", C.c_int),
         ("kT_window_around_singularity", C.c_double * 2),
+        ("noneq_integration_pts_density", C.c_int * 8),
         ("flatband_dos_integration_pts", C.c_int),
         ("segment_length", C.c_int),
     ]

@@ -253,6 +253,14 @@ int Tube::init(const negf_tube_params& prm)
     intercuts.assign(np > 1 ? np - 1 : 0, 0.0);
     for (int c = 0; c + 1 < np; ++c) intercuts[c] = p.noneq_percent_intercuts[c];
     density.clear();
+    // noneq_integration_pts_density is read only with adaptive limits
+    // (NEGF_Common.cpp:527-539); when given it is used from the first correction
+    if (p.flag_adaptive_integration_limits) {
+        bool given = false;
+        for (int c = 0; c < np; ++c) given |= p.noneq_integration_pts_density[c] != 0;
+        if (given)
+            for (int c = 0; c < np; ++c) density.push_back((double)p.noneq_integration_pts_density[c]);
+    }
 
     NEGF_HIP_TRY(hipStreamCreate(&stream));
     int rc;
@@ -432,6 +440,9 @@ int Tube::build_noneq_plan()
 
 // Find_NonEq_Percent_Intercuts_Adaptively (NEGF_Common.cpp:2062-2185); the
 // point densities per kT are fixed at the first call (or given as input).
+// The reference keeps them in an int vector (noneq_integration_pts_density,
+// NEGF_Common.H:404), so a density computed here is truncated toward zero
+// when it is stored, as there.
 void Tube::adapt_intercuts()
 {
     const double kT0 = L.kT_contact[0];
@@ -440,11 +451,12 @@ void Tube::adapt_intercuts()
     const double E_max = (L.E_rightmost.real() - mu[0]) / kT0;
     const double span = E_max - E_min;
     auto even_round = [](double x) { return (int)(std::round(x / 2) * 2); };
+    auto as_int = [](double x) { return (double)(int)x; };
     if (np == 3) {
         if (density.empty())
-            density = {pts[0] / ((intercuts[0] / 100.) * span),
-                       pts[1] / (((intercuts[1] - intercuts[0]) / 100.) * span),
-                       pts[2] / (((100. - intercuts[1]) / 100.) * span)};
+            density = {as_int(pts[0] / ((intercuts[0] / 100.) * span)),
+                       as_int(pts[1] / (((intercuts[1] - intercuts[0]) / 100.) * span)),
+                       as_int(pts[2] / (((100. - intercuts[1]) / 100.) * span))};
         const double E_lo = std::max(E_star - p.kT_window_around_singularity[0], E_min);
         const double E_hi = std::min(E_star + p.kT_window_around_singularity[1], E_max);
         intercuts[0] = 100 * (E_lo - E_min) / span;
@@ -454,7 +466,8 @@ void Tube::adapt_intercuts()
         pts[2] = even_round((E_max - E_hi) * density[2]);
     } else if (np == 2) {
         if (density.empty())
-            density = {pts[0] / ((intercuts[0] / 100.) * span), pts[1] / (((100 - intercuts[0]) / 100.) * span)};
+            density = {as_int(pts[0] / ((intercuts[0] / 100.) * span)),
+                       as_int(pts[1] / (((100 - intercuts[0]) / 100.) * span))};
         const double E_hi = E_star + p.kT_window_around_singularity[1];
         intercuts[0] = 100 * (E_hi - E_min) / span;
         pts[0] = even_round((E_hi - E_min) * density[0]);

@@ -164,6 +164,10 @@ typedef struct {
     int flag_adaptive_integration_limits;
     int integrand_correction_interval;
     double kT_window_around_singularity[2];
+    int noneq_integration_pts_density[8]; /* points per kT of each noneq    */
+                                    /* path (NEGF_Common.cpp:537-539), read */
+                                    /* with adaptive limits; all 0: not    */
+                                    /* given, set from the first correction */
     int flatband_dos_integration_pts;
     int segment_length;             /* RGF segment length (0: default); no */
                                     /* reference counterpart (performance)  */

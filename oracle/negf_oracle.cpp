@@ -750,15 +750,18 @@ int orc_adaptive_intercuts(orc_tube *t, const double *U, const double Uc[2])
     double E_star = (t->E_at_max - t->mu[0]) / kT0;
     double E_min = (L.E_contour_right.re - t->mu[0]) / kT0;
     double E_max = (L.E_rightmost.re - t->mu[0]) / kT0;
+    /* noneq_integration_pts_density is an amrex::Vector<int>
+     * (NEGF_Common.H:404): the computed densities are truncated on assignment */
     double *d = t->noneq_density;
+    auto as_int = [](double x) { return (double)(int)x; };
     if (t->num_noneq_paths == 3) {
         if (!t->has_density) {
-            d[0] = t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min));
-            d[1] = t->noneq_pts[1] /
-                   (((t->noneq_intercuts[1] - t->noneq_intercuts[0]) / 100.) *
-                    (E_max - E_min));
-            d[2] = t->noneq_pts[2] /
-                   (((100. - t->noneq_intercuts[1]) / 100.) * (E_max - E_min));
+            d[0] = as_int(t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min)));
+            d[1] = as_int(t->noneq_pts[1] /
+                          (((t->noneq_intercuts[1] - t->noneq_intercuts[0]) / 100.) *
+                           (E_max - E_min)));
+            d[2] = as_int(t->noneq_pts[2] /
+                          (((100. - t->noneq_intercuts[1]) / 100.) * (E_max - E_min)));
             t->has_density = 1;
         }
         double E_lower = std::max(E_star - t->kT_window[0], E_min);
@@ -770,9 +773,9 @@ int orc_adaptive_intercuts(orc_tube *t, const double *U, const double Uc[2])
         t->noneq_pts[2] = int(std::round(((E_max - E_upper) * d[2]) / 2) * 2);
     } else if (t->num_noneq_paths == 2) {
         if (!t->has_density) {
-            d[0] = t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min));
-            d[1] = t->noneq_pts[1] /
-                   (((100 - t->noneq_intercuts[0]) / 100.) * (E_max - E_min));
+            d[0] = as_int(t->noneq_pts[0] / ((t->noneq_intercuts[0] / 100.) * (E_max - E_min)));
+            d[1] = as_int(t->noneq_pts[1] /
+                          (((100 - t->noneq_intercuts[0]) / 100.) * (E_max - E_min)));
             t->has_density = 1;
         }
         double E_upper = E_star + t->kT_window[1];

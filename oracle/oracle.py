@@ -244,7 +244,7 @@ def make_tube(N, betas, degens, gamma=2.5, mu=(0.0, 0.0), T=(298.0, 298.0),
               tails=(14.0, 14.0), eq_pts=(30, 30, 30), noneq_pts=(100,),
               intercuts=(), spin=2, use_decimation=0, decimation_layers=10,
               decimation_max_iter=30, decimation_rel_error=0,
-              kT_window=(1.0, 1.0)):
+              kT_window=(1.0, 1.0), density=None):
     t = orc_tube()
     t.N = N
     t.nmodes = len(betas)
@@ -274,6 +274,10 @@ def make_tube(N, betas, degens, gamma=2.5, mu=(0.0, 0.0), T=(298.0, 298.0),
     t.decimation_max_iter = decimation_max_iter
     t.decimation_rel_error = decimation_rel_error
     t.kT_window[0], t.kT_window[1] = kT_window
+    if density is not None:   # noneq_integration_pts_density given (ints)
+        t.has_density = 1
+        for i, d in enumerate(density):
+            t.noneq_density[i] = int(d)
     return t
 
 

@@ -41,7 +41,8 @@ def pair(num_unitcells=100, num_modes=1, mu=(-1.0, -1.0), T=(298.0, 298.0),
                      use_decimation=extra.get("use_decimation", 0),
                      decimation_layers=extra.get("decimation_layers", 10),
                      decimation_max_iter=extra.get("decimation_max_iter", 30),
-                     decimation_rel_error=extra.get("decimation_rel_error", 0))
+                     decimation_rel_error=extra.get("decimation_rel_error", 0),
+                     density=extra.get("noneq_integration_pts_density"))
     return tube, ot
 
 
@@ -112,13 +113,14 @@ def test_contact_update_rebuilds_paths():
     assert npts == npo and rel(Q, Qo) < Q_TOL
 
 
-@pytest.mark.parametrize("npaths", [2, 3])
-def test_adaptive_intercuts_match_oracle(npaths):
+@pytest.mark.parametrize("npaths,density", [(2, False), (3, False), (2, True), (3, True)])
+def test_adaptive_intercuts_match_oracle(npaths, density):
     pts = (60, 80, 60)[:npaths]
     cuts = (40.0, 60.0)[:npaths - 1]
     mu = (-1.0, -1.15)
+    extra = {"noneq_integration_pts_density": (9, 13, 7)[:npaths]} if density else {}
     tube, ot = pair(num_unitcells=25, mu=mu, noneq_pts=pts, intercuts=cuts,
-                    flag_adaptive_integration_limits=1, integrand_correction_interval=2)
+                    flag_adaptive_integration_limits=1, integrand_correction_interval=2, **extra)
     U = gaussian_bumps(tube.N, seed=9, amp=0.25)
     Uc = (U[0], U[-1])
     tube.set_potential(U, Uc)

@@ -126,14 +126,35 @@ def test_adaptive_intercuts_window_clamped_hand_derived():
     """Windows wider than the integration range clamp to E_min and E_max
     (NEGF_Common.cpp:2134-2140): cuts (0, 100); the middle path's points are
     (E_max - E_min) * d1 rounded to even with d1 = 40 / (0.5 (E_max - E_min))
-    from the first call, i.e. 80; the outer paths get 0."""
+    per kT from the first call, stored in the reference's int vector
+    (NEGF_Common.H:404), i.e. truncated: E_max - E_min = 28 + 0.2/kT = 35.79 kT,
+    d1 = int(2.235) = 2, points 2 round(35.79) = 72 (80 without the
+    truncation); the outer paths get 0."""
     N = 32
     U = np.zeros(N)
+    kT = KB * 298.0
     t = _intercut_tube(N, (0.0, -0.2), (20, 40, 20), (25.0, 75.0), (1000.0, 1000.0))
     O.adaptive_intercuts(t, U, (0.0, 0.0))
     pts, cuts = _state(t)
-    assert pts == [0, 80, 0]
+    span_kT = 28 + 0.2 / kT
+    assert int(40 / (0.5 * span_kT)) == 2
+    assert pts == [0, 2 * round(span_kT * 2 / 2), 0] == [0, 72, 0]
     assert cuts[0] == 0.0 and abs(cuts[1] - 100.0) < 1e-12
+
+
+def test_adaptive_intercuts_given_density():
+    """noneq_integration_pts_density given as input (NEGF_Common.cpp:537-539)
+    replaces the densities the first correction would compute: with (3, 5, 3)
+    points per kT and the clamped window of the test above, the middle path
+    gets 2 round(35.79 * 5 / 2) = 178 points."""
+    N = 32
+    U = np.zeros(N)
+    kT = KB * 298.0
+    t = O.make_tube(N, [BETA], [2], mu=(0.0, -0.2), eq_pts=(30, 30, 64), noneq_pts=(20, 40, 20),
+                    intercuts=(25.0, 75.0), eta=1e-5, kT_window=(1000.0, 1000.0), density=(3, 5, 3))
+    O.adaptive_intercuts(t, U, (0.0, 0.0))
+    pts, _ = _state(t)
+    assert pts == [0, 2 * round((28 + 0.2 / kT) * 5 / 2), 0] == [0, 178, 0]
 
 
 def test_adaptive_intercuts_interior_window_hand_derived():
@@ -170,10 +191,13 @@ def test_adaptive_intercuts_interior_window_hand_derived():
     hi = min(E_star + kT, Erm)
     assert abs(cuts[0] - 100 * (lo - Ecr) / span) < 1e-9
     assert abs(cuts[1] - 100 * (hi - Ecr) / span) < 1e-9
-    lens0 = [0.25 * span, 0.5 * span, 0.25 * span]
-    want = [2 * round((lo - Ecr) * npts[0] / lens0[0] / 2),
-            2 * round((hi - lo) * npts[1] / lens0[1] / 2),
-            2 * round((Erm - hi) * npts[2] / lens0[2] / 2)]
+    # densities per kT of the first call's paths, truncated to int as the
+    # reference stores them (NEGF_Common.H:404)
+    fr = [0.25, 0.5, 0.25]
+    d = [int(npts[i] / (fr[i] * span / kT)) for i in range(3)]
+    want = [2 * round((lo - Ecr) / kT * d[0] / 2),
+            2 * round((hi - lo) / kT * d[1] / 2),
+            2 * round((Erm - hi) / kT * d[2] / 2)]
     assert pts == want
     assert all(p % 2 == 0 for p in pts)
 
@@ -182,19 +206,20 @@ def test_adaptive_intercuts_densities_frozen_at_first_call():
     """The point densities per kT are set at the first correction and kept
     (NEGF_Common.cpp:2078-2100): after the bias grows from 0.2 to 0.4 V the
     clamped middle path covers a range 28 + 0.4/kT kT wide at the first
-    call's density 40 / (0.5 (28 + 0.2/kT)), not 80 points again."""
+    call's density int(40 / (0.5 (28 + 0.2/kT))) = 2 per kT, not 72 points
+    again."""
     N = 32
     U = np.zeros(N)
     kT = KB * 298.0
     t = _intercut_tube(N, (0.0, -0.2), (20, 40, 20), (25.0, 75.0), (1000.0, 1000.0))
     O.adaptive_intercuts(t, U, (0.0, 0.0))
-    assert _state(t)[0] == [0, 80, 0]
+    assert _state(t)[0] == [0, 72, 0]
     t.mu[1] = -0.4
     O.adaptive_intercuts(t, U, (0.0, 0.0))
-    d1 = 40 / (0.5 * (28 + 0.2 / kT))
+    d1 = int(40 / (0.5 * (28 + 0.2 / kT)))
     want_mid = 2 * round((28 + 0.4 / kT) * d1 / 2)
     assert _state(t)[0] == [0, want_mid, 0]
-    assert want_mid != 80
+    assert want_mid != 72
 
 
 def test_adaptive_intercuts_two_paths_hand_derived():
@@ -212,8 +237,8 @@ def test_adaptive_intercuts_two_paths_hand_derived():
     E_min = -0.2 / kT - 14
     E_up = E_star + 1e6
     assert abs(cuts[0] - 100 * (E_up - E_min) / span_kT) < 1e-6 * abs(cuts[0])
-    d0 = 30 / (0.5 * span_kT)
-    d1 = 30 / (0.5 * span_kT)
+    d0 = int(30 / (0.5 * span_kT))
+    d1 = int(30 / (0.5 * span_kT))
     assert pts[0] == 2 * round((E_up - E_min) * d0 / 2)
     assert pts[1] == 2 * round((14 - E_up) * d1 / 2)
     assert pts[1] < 0 and pts[1] % 2 == 0
