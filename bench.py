@@ -184,6 +184,32 @@ def batch_leg(dev, steps=10):
         out[f"energy_site_per_s_{key}"] = es / (ms * 1e-3)
     if not np.all(np.isfinite(dQ.cpu().numpy())):
         raise RuntimeError("non-finite batched charge")
+    # every tube is biased (V > 0): 128 contour chains at 31 flop and 128
+    # real-axis chains at 63 flop per mode and site
+    if abs(es - 2 * 256 * n) > 0.5:
+        raise RuntimeError(f"unexpected chain count: {es} energy-sites for {n} sites")
+    flop = float(n) * 2 * (128 * 31 + 128 * 63)
+    ms_solve = out["ms_per_solve_graph"]
+    out["flop_per_solve"] = flop
+    out["roofline"] = {"bound": "fp64_valu", "achieved": flop / (ms_solve * 1e-3) / 1e12, "peak": 78.6,
+                       "unit": "TFLOP/s", "frac": flop / (ms_solve * 1e-3) / 1e12 / 78.6}
+    # one SCF iteration over all tubes with one Broyden over the 317972 sites
+    # (negf_multi_scf): electrostatic stand-in, batched solve, Broyden,
+    # replayed from a captured graph; timed over scf_iters iterations
+    scf_iters = 8
+    ms_scf = negf.MultiScf(batch, Broyden_fraction=0.1, Broyden_max_norm=1e-12, eps_r=3.9,
+                           R_ring_nm=0.665, R_image_nm=10.0)
+    ms_scf.set_external_potential(np.concatenate([smooth_potential(t.N, seed=100 + i, lo=-0.3, hi=0.3)
+                                                  for i, t in enumerate(tubes)]))
+    U, Q, it, nrm, ms_it = ms_scf.bias_point(np.zeros(len(tubes)), max_iters=scf_iters)
+    if it != scf_iters or not np.all(np.isfinite(U)):
+        raise RuntimeError("multi-tube SCF iterations failed")
+    out["scf"] = {"what": "SCF iteration over all 64 tubes (one Broyden over all sites; dense 1/r-with-image "
+                          "stand-in within each tube; batched NEGF solve), hipGraph replay",
+                  "iterations_timed": it, "ms_per_iteration": ms_it,
+                  "scf_iterations_per_s": 1e3 / ms_it, "ms_negf_solve": ms_solve,
+                  "broyden_norm_after": nrm}
+    ms_scf.close()
     batch.close()
     for t in tubes:
         t.close()

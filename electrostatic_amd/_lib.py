@@ -83,6 +83,13 @@ SIGNATURES = {
     "negf_last_error": (C.c_char_p, []),
     "negf_version": (C.c_int, []),
     "negf_max_sites": (C.c_int, []),
+    "negf_multi_scf_create": (C.c_int, [vp, vp, C.c_int, vp, vp]),
+    "negf_multi_scf_destroy": (C.c_int, [vp]),
+    "negf_multi_scf_num_sites": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+    "negf_multi_scf_set_external_potential": (C.c_int, [vp, vp]),
+    "negf_multi_scf_use_graph": (C.c_int, [vp, C.c_int]),
+    "negf_multi_scf_bias_point": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),
+    "negf_multi_scf_get_input_charge": (C.c_int, [vp, vp]),
     "negf_rgf_diag_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp]),
     "negf_rgf_diag_accumulate_noalloc": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp,
                                                    vp]),

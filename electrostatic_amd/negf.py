@@ -354,6 +354,64 @@ class Broyden:
         return self.lib.negf_broyden_get_step(self.h)
 
 
+class MultiScf:
+    """SCF over the tubes of a batch (any lengths) with one Broyden system over
+    their concatenated sites (negf_multi_scf_*)."""
+
+    def __init__(self, batch, params=None, **keys):
+        self.lib = _lib.load()
+        self.batch = batch
+        self.T = len(batch.tubes)
+        self.params = params if params is not None else scf_params(**keys)
+        arr = (C.c_void_p * self.T)(*[t.h for t in batch.tubes])
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_multi_scf_create(batch.h, arr, self.T, C.byref(self.params), C.byref(h)))
+        self.h = h
+        S = C.c_longlong()
+        _lib.check(self.lib.negf_multi_scf_num_sites(h, C.byref(S)))
+        self.S = S.value
+
+    def close(self):
+        if self.h:
+            self.lib.negf_multi_scf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_external_potential(self, U_ext):
+        U = np.ascontiguousarray(U_ext, np.float64)
+        if U.size != self.S:
+            raise ValueError(f"U_ext needs {self.S} values, got {U.size}")
+        self._U_ext = U.reshape(self.S)
+        _lib.check(self.lib.negf_multi_scf_set_external_potential(self.h, _dp(self._U_ext)))
+
+    def use_graph(self, on=True):
+        _lib.check(self.lib.negf_multi_scf_use_graph(self.h, int(on)))
+
+    def bias_point(self, V_gate, max_iters=0):
+        """Returns (U, Q, iterations, final norm, ms per iteration)."""
+        V = np.ascontiguousarray(V_gate, np.float64)
+        if V.size != self.T:
+            raise ValueError(f"V_gate needs one value per tube ({self.T}), got {V.size}")
+        U = np.zeros(self.S)
+        Q = np.zeros(self.S)
+        it = C.c_int()
+        nrm = C.c_double()
+        ms = C.c_double()
+        _lib.check(self.lib.negf_multi_scf_bias_point(self.h, _dp(V), int(max_iters), _dp(U), _dp(Q),
+                                                      C.byref(it), C.byref(nrm), C.byref(ms)))
+        return U, Q, it.value, nrm.value, ms.value
+
+    def input_charge(self):
+        n = np.zeros(self.S)
+        _lib.check(self.lib.negf_multi_scf_get_input_charge(self.h, _dp(n)))
+        return n
+
+
 def scf_params(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0, Broyden_fraction=0.1,
                Broyden_max_norm=1e-6, Broyden_norm_type=0, Broyden_threshold_maxstep=200,
                reset_with_previous_charge_distribution=1, NS_initial_deposit_value=0.0):

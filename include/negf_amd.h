@@ -343,6 +343,37 @@ int negf_batch_scf_run(negf_batch_scf *s, const double *V_gate, double *U_out,
                        double *Q_out, int *iters, double *final_norms,
                        int *total_iterations);
 
+/* Self-consistent loop over the tubes of batch b (any lengths, same order)
+ * with ONE Broyden system over their concatenated sites, as the reference's
+ * transport driver runs several nanostructures: each tube's Solve_NEGF fills
+ * its block of n_curr_out at its site offset (Transport.cpp:417-431,
+ * Broyden_Parallel_General.cpp:42-66) and one Broyden update covers all of
+ * them (Transport.cpp:435); Broyden_Norm is the maximum over all sites.  The
+ * electrostatic stand-in is negf_scf's dense 1/r-with-image kernel within
+ * each tube (applied as a direct sum, not stored), with a per-tube metal
+ * voltage V_gate[t] and the external potential U_ext.  One iteration
+ * (stand-in, batched NEGF solve, three Broyden launches) is replayed from a
+ * captured hipGraph.  Arrays are host, concatenated in batch order
+ * (negf_batch_offsets); S = total sites. */
+typedef struct negf_multi_scf negf_multi_scf;
+int negf_multi_scf_create(negf_batch *b, negf_tube *const *tubes, int ntubes,
+                          const negf_scf_params *p, negf_multi_scf **out);
+int negf_multi_scf_destroy(negf_multi_scf *s);
+int negf_multi_scf_num_sites(const negf_multi_scf *s, long long *S);
+int negf_multi_scf_set_external_potential(negf_multi_scf *s, const double *U_ext);
+int negf_multi_scf_use_graph(negf_multi_scf *s, int on);
+/* One bias point: iterate until Broyden_Norm <= Broyden_max_norm (or for
+ * max_iters iterations when max_iters > 0), then Reset_ForNextBiasStep
+ * (Transport.cpp:713-717).  U_out, Q_out (optional, [S]): the last
+ * iteration's potential and NEGF charge; ms_per_iter: mean device time of
+ * one iteration. */
+int negf_multi_scf_bias_point(negf_multi_scf *s, const double *V_gate,
+                              int max_iters, double *U_out, double *Q_out,
+                              int *iters, double *final_norm,
+                              double *ms_per_iter);
+/* Broyden input charge n_in ([S]). */
+int negf_multi_scf_get_input_charge(const negf_multi_scf *s, double *n_in);
+
 /* ------------------------------------------------------------------------
  * Dense-block chains on FP64 MFMA.  The reference declares this path but
  * leaves it stubbed: the 2-D MatrixBlock multiply/inverse/dagger are empty
