@@ -165,12 +165,15 @@ class Batch {
             changed |= tubes[i]->paths_version != versions[i];
         }
         if (changed && (rc = build())) return rc;
-        // the batch stream follows the caller's stream and the tubes' streams
+        // the batch stream follows the caller's stream and any single-tube
+        // solve still in flight on a tube's own stream
         NEGF_HIP_TRY(hipEventRecord(ev_in, st));
         NEGF_HIP_TRY(hipStreamWaitEvent(stream, ev_in, 0));
         for (Tube* t : tubes) {
+            if (!t->stream_pending) continue;
             NEGF_HIP_TRY(hipEventRecord(ev_in, t->stream));
             NEGF_HIP_TRY(hipStreamWaitEvent(stream, ev_in, 0));
+            t->stream_pending = false;
         }
         if (timing) NEGF_HIP_TRY(hipEventRecord(ev0, stream));
         if (use_graph) {

@@ -69,6 +69,8 @@ class Tube {
     int n_eq = 0, n_eq_pts = 0, n_neq = 0, n_neq_E = 0;
 
     hipStream_t stream = nullptr;
+    // work enqueued on stream by solve() that a batch solve must still wait for
+    bool stream_pending = false;
     std::vector<double> U_host;
     double Uc_host[2] = {0, 0};
     DevBuf d_U, d_Uc, d_zero, d_rho0, d_eq, d_neq, d_Q;

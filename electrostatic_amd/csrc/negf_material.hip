@@ -616,6 +616,7 @@ int Tube::solve(int iter, double* dQ, hipStream_t st)
     hipLaunchKernelGGL(combine_charge_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, N, d_eq.as<double>(),
                        d_neq.as<double>(), d_rho0.as<double>(), dQ);
     NEGF_HIP_TRY(hipGetLastError());
+    stream_pending = true;
     if (st != stream) {
         hipEvent_t ev;
         NEGF_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));

@@ -128,27 +128,33 @@ __global__ __launch_bounds__(kMsThreads) void big_broyden_pass1(BigBroyden b, in
     }
 }
 
-// Pass 2 (one workgroup): the partials of pass 1 added in block order; the
-// coefficients of the history update; the step counter advanced.
+// Pass 2 (one workgroup): the partials of pass 1 added in a fixed order
+// (thread-strided over the blocks, then the fixed workgroup reduction), the
+// coefficients of the history update, and the step counter advanced.
 __global__ __launch_bounds__(kMsThreads) void big_broyden_pass2(BigBroyden b, int nblocks)
 {
+    __shared__ double sh[kMsThreads / 64];
+    __shared__ double tot[4];
     const int m = *b.step - 1;
     const int stride = 4 + 2 * b.max_step;
     const int nq = 4 + 2 * (m > 1 ? m - 1 : 0);
-    __shared__ double tot[4];
-    for (int q = threadIdx.x; q < nq; q += kMsThreads) {
+    for (int q = 0; q < nq; ++q) {
         double acc = 0.0;
-        for (int k = 0; k < nblocks; ++k) {
+        for (int k = threadIdx.x; k < nblocks; k += kMsThreads) {
             const double x = b.part[(size_t)k * stride + q];
             acc = (q == 1) ? fmax(acc, x) : acc + x;
         }
-        if (q < 4) tot[q] = acc;
-        else {
-            const int j = 1 + (q - 4) / 2;
-            if ((q - 4) % 2 == 0)
-                b.coef[4 + j] = acc;                  // a_j = V_j . dF
-            else
-                b.coef[4 + b.max_step + 1 + j] = acc; // b_j = V_j . F (j < m)
+        acc = (q == 1) ? block_reduce<true>(acc, sh) : block_reduce<false>(acc, sh);
+        if (threadIdx.x == 0) {
+            if (q < 4) {
+                tot[q] = acc;
+            } else {
+                const int j = 1 + (q - 4) / 2;
+                if ((q - 4) % 2 == 0)
+                    b.coef[4 + j] = acc;                   // a_j = V_j . dF
+                else
+                    b.coef[4 + b.max_step + 1 + j] = acc;  // b_j = V_j . F (j < m)
+            }
         }
     }
     __syncthreads();
@@ -193,42 +199,99 @@ __global__ __launch_bounds__(kMsThreads) void big_broyden_pass3(BigBroyden b, do
 
 __global__ void big_broyden_first_step(int* step) { *step = 1; }
 
-// Electrostatic stand-in, rows [row0, row0 + 256) of one tube per workgroup:
-// U_i = U_ext,i - V_t - sum_j kt[i % 2][i - j + D] n_j, the charge staged in
-// LDS 256 sites at a time; the sum runs over j in increasing order.  The
-// potential goes to the concatenated U, the tube's own potential and its
-// contact potentials U_0, U_{N-1}.
-struct MsBlock {
-    int tube, row0;
+// Electrostatic stand-in: U_i = U_ext,i - V_t - sum_j K(i, j) n_j within each
+// tube, K(i, j) = kt[i % 2][i - j + D].  The (rows x columns) plane of each
+// tube is cut into tiles of MS_ROWS x MS_TILE; a workgroup sums one tile's
+// columns for its rows into a partial row (partial[c][i] for column tile c),
+// and a second launch adds the partials of a row in column-tile order (a
+// fixed order) and forms U.  Within a tile, thread (q, p) (p = thread % 2)
+// owns the MS_RPT rows row0 + p + 2 (MS_RPT q + r) of parity p, whose kernel
+// values for 8 consecutive columns lie in a window of 2 MS_RPT + 6
+// consecutive entries of table p: per group of 8 columns a thread loads that
+// window and the 8 charges once from LDS and does 8 MS_RPT FMAs.  Columns
+// are staged MS_COLS at a time (charges and the two table stretches they
+// need, padded by one double every 16 against LDS bank conflicts).
+#define MS_RPT 8
+#define MS_ROWS (kMsThreads * MS_RPT)
+#define MS_TILE 2048
+#define MS_COLS 256
+#define MS_SPAN (MS_ROWS + MS_COLS - 1)
+#define MS_PAD(e) ((e) + ((e) >> 4))
+struct MsTile {
+    int tube, row0, col0;
+    long long part;  // partial row of this tile's column block: partial + part + i
 };
-__global__ __launch_bounds__(kMsThreads) void multi_potential_kernel(
-    const MsBlock* __restrict__ blocks, const long long* __restrict__ off, const double* __restrict__ kt, int D,
-    const double* __restrict__ n, const double* __restrict__ Uext, const double* __restrict__ V,
-    double* const* __restrict__ dU, double* const* __restrict__ dUc, double* __restrict__ Uall)
+struct MsRows {
+    int tube, row0, ntiles;
+    long long part;  // the tube's partial rows: partial + part + c N + i
+};
+__global__ __launch_bounds__(kMsThreads) void multi_potential_tile_kernel(
+    const MsTile* __restrict__ tiles, const long long* __restrict__ off, const double* __restrict__ kt, int D,
+    const double* __restrict__ n, double* __restrict__ partial)
 {
-    __shared__ double ns[kMsThreads];
-    const MsBlock bk = blocks[blockIdx.x];
-    const long long o = off[bk.tube];
-    const int N = (int)(off[bk.tube + 1] - o);
-    const int i = bk.row0 + threadIdx.x;
-    const double* __restrict__ k = kt + (size_t)(i & 1) * (2 * D + 1) + D + i;  // k[-j] = K(i, j)
-    double acc = 0.0;
-    for (int j0 = 0; j0 < N; j0 += kMsThreads) {
+    __shared__ double ns[MS_COLS];
+    __shared__ double ks[2][MS_PAD(MS_SPAN) + 1];
+    const MsTile tl = tiles[blockIdx.x];
+    const long long o = off[tl.tube];
+    const int N = (int)(off[tl.tube + 1] - o);
+    const int jend = min(N, tl.col0 + MS_TILE);
+    const int p = threadIdx.x & 1, q = threadIdx.x >> 1;
+    const int ib = tl.row0 + p + 2 * MS_RPT * q;  // first row of this thread (row0 even)
+    double acc[MS_RPT];
+#pragma unroll
+    for (int r = 0; r < MS_RPT; ++r) acc[r] = 0.0;
+    for (int j0 = tl.col0; j0 < jend; j0 += MS_COLS) {
+        // table entries d = dlo + e, e in [0, MS_SPAN), for both parities
+        const int dlo = tl.row0 - j0 - (MS_COLS - 1);
         __syncthreads();
-        ns[threadIdx.x] = (j0 + (int)threadIdx.x < N) ? n[o + j0 + threadIdx.x] : 0.0;
+        ns[threadIdx.x] = (j0 + (int)threadIdx.x < jend) ? n[o + j0 + threadIdx.x] : 0.0;
+        for (int e = threadIdx.x; e < 2 * MS_SPAN; e += kMsThreads) {
+            const int pp = e >= MS_SPAN, ee = e - pp * MS_SPAN;
+            const int d = dlo + ee;
+            ks[pp][MS_PAD(ee)] = (d >= -D && d <= D) ? kt[(size_t)pp * (2 * D + 1) + d + D] : 0.0;
+        }
         __syncthreads();
-        if (i < N) {
-            const int jn = min(kMsThreads, N - j0);
-            for (int jj = 0; jj < jn; ++jj) acc = fma(k[-(j0 + jj)], ns[jj], acc);
+        const double* __restrict__ kp = ks[p];
+        // entry of d = ib - j0 - jg - 7 + k is e = ib - row0 - jg + MS_COLS - 8 + k
+        const int e0 = ib - tl.row0 + MS_COLS - 8;
+        for (int jg = 0; jg < MS_COLS; jg += 8) {
+            double w[2 * MS_RPT + 6], x[8];
+#pragma unroll
+            for (int k = 0; k < 2 * MS_RPT + 6; ++k) w[k] = kp[MS_PAD(e0 - jg + k)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) x[u] = ns[jg + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int r = 0; r < MS_RPT; ++r) acc[r] = fma(w[2 * r - u + 7], x[u], acc[r]);
         }
     }
-    if (i < N) {
-        const double u = Uext[o + i] - V[bk.tube] - acc;
-        Uall[o + i] = u;
-        dU[bk.tube][i] = u;
-        if (i == 0) dUc[bk.tube][0] = u;
-        if (i == N - 1) dUc[bk.tube][1] = u;
+#pragma unroll
+    for (int r = 0; r < MS_RPT; ++r) {
+        const int i = ib + 2 * r;
+        if (i < N) partial[tl.part + i] = acc[r];
     }
+}
+
+// U from the partial rows (column tiles in order), into the concatenated U,
+// the tube's own potential and its contact potentials U_0, U_{N-1}.
+__global__ __launch_bounds__(kMsThreads) void multi_potential_finish(
+    const MsRows* __restrict__ rows, const long long* __restrict__ off, const double* __restrict__ partial,
+    const double* __restrict__ Uext, const double* __restrict__ V, double* const* __restrict__ dU,
+    double* const* __restrict__ dUc, double* __restrict__ Uall)
+{
+    const MsRows rw = rows[blockIdx.x];
+    const long long o = off[rw.tube];
+    const int N = (int)(off[rw.tube + 1] - o);
+    const int i = rw.row0 + threadIdx.x;
+    if (i >= N) return;
+    double acc = 0.0;
+    for (int c = 0; c < rw.ntiles; ++c) acc += partial[rw.part + (long long)c * N + i];
+    const double u = Uext[o + i] - V[rw.tube] - acc;
+    Uall[o + i] = u;
+    dU[rw.tube][i] = u;
+    if (i == 0) dUc[rw.tube][0] = u;
+    if (i == N - 1) dUc[rw.tube][1] = u;
 }
 
 class MultiScf {
@@ -260,7 +323,9 @@ class MultiScf {
         int Nmax = 0;
         const double acc = tubes[0]->t.p.acc;
         std::vector<double*> hU(T), hUc(T);
-        std::vector<MsBlock> blocks;
+        std::vector<MsTile> tiles;
+        std::vector<MsRows> rows;
+        long long pbase = 0;
         for (int t = 0; t < T; ++t) {
             const Tube& tu = tubes[t]->t;
             off[t + 1] = off[t] + tu.N;
@@ -275,10 +340,16 @@ class MultiScf {
             Nmax = std::max(Nmax, tu.N);
             hU[t] = tu.d_U.as<double>();
             hUc[t] = tu.d_Uc.as<double>();
-            for (int r = 0; r < tu.N; r += kMsThreads) blocks.push_back(MsBlock{t, r});
+            const int C = (tu.N + MS_TILE - 1) / MS_TILE;
+            for (int r = 0; r < tu.N; r += MS_ROWS)
+                for (int c = 0; c < C; ++c)
+                    tiles.push_back(MsTile{t, r, c * MS_TILE, pbase + (long long)c * tu.N});
+            for (int r = 0; r < tu.N; r += kMsThreads) rows.push_back(MsRows{t, r, C, pbase});
+            pbase += (long long)C * tu.N;
         }
         S = off[T];
-        nblk_pot = (int)blocks.size();
+        ntiles = (int)tiles.size();
+        nrows = (int)rows.size();
         // Kernel table: ring r sits at the axial position (acc / 4) (3 r + 4 -
         // (r % 2)) plus the tube's centring (ring_positions of the tube), so
         // x_i - x_j = (acc / 4) (3 d - (i % 2) + (j % 2)) with d = i - j.
@@ -298,7 +369,9 @@ class MultiScf {
         const size_t hist = (size_t)(M + 1) * S;
         if ((rc = d_kt.ensure(sizeof(double) * kt.size()))) return rc;
         if ((rc = d_off.ensure(sizeof(long long) * (T + 1)))) return rc;
-        if ((rc = d_blocks.ensure(sizeof(MsBlock) * blocks.size()))) return rc;
+        if ((rc = d_tiles.ensure(sizeof(MsTile) * tiles.size()))) return rc;
+        if ((rc = d_rows.ensure(sizeof(MsRows) * rows.size()))) return rc;
+        if ((rc = d_partial.ensure(sizeof(double) * pbase))) return rc;
         if ((rc = d_ptrU.ensure(sizeof(double*) * T))) return rc;
         if ((rc = d_ptrUc.ensure(sizeof(double*) * T))) return rc;
         if ((rc = d_Uext.ensure(sizeof(double) * S))) return rc;
@@ -313,7 +386,8 @@ class MultiScf {
         if ((rc = h_out.ensure(sizeof(double) * 4))) return rc;
         NEGF_HIP_TRY(hipMemcpy(d_kt.p, kt.data(), sizeof(double) * kt.size(), hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_off.p, off.data(), sizeof(long long) * (T + 1), hipMemcpyHostToDevice));
-        NEGF_HIP_TRY(hipMemcpy(d_blocks.p, blocks.data(), sizeof(MsBlock) * blocks.size(), hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_tiles.p, tiles.data(), sizeof(MsTile) * tiles.size(), hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_rows.p, rows.data(), sizeof(MsRows) * rows.size(), hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_ptrU.p, hU.data(), sizeof(double*) * T, hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_ptrUc.p, hUc.data(), sizeof(double*) * T, hipMemcpyHostToDevice));
         if ((rc = fill_device(d_Uext.as<double>(), S, 0.0, st))) return rc;
@@ -354,14 +428,22 @@ class MultiScf {
         return NEGF_OK;
     }
 
+    // U = U_ext - V - K n_in for every tube, into the tubes
+    int enqueue_potential()
+    {
+        hipLaunchKernelGGL(multi_potential_tile_kernel, dim3(ntiles), dim3(kMsThreads), 0, st, d_tiles.as<MsTile>(),
+                           d_off.as<long long>(), d_kt.as<double>(), D, bro().n_in, d_partial.as<double>());
+        hipLaunchKernelGGL(multi_potential_finish, dim3(nrows), dim3(kMsThreads), 0, st, d_rows.as<MsRows>(),
+                           d_off.as<long long>(), d_partial.as<double>(), d_Uext.as<double>(), d_V.as<double>(),
+                           d_ptrU.as<double*>(), d_ptrUc.as<double*>(), d_U.as<double>());
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
     int enqueue_iteration()
     {
-        hipLaunchKernelGGL(multi_potential_kernel, dim3(nblk_pot), dim3(kMsThreads), 0, st,
-                           d_blocks.as<MsBlock>(), d_off.as<long long>(), d_kt.as<double>(), D, bro().n_in,
-                           d_Uext.as<double>(), d_V.as<double>(), d_ptrU.as<double*>(), d_ptrUc.as<double*>(),
-                           d_U.as<double>());
-        NEGF_HIP_TRY(hipGetLastError());
-        int rc = batch_enqueue(batch, d_Q.as<double>(), st, nullptr);
+        int rc = enqueue_potential();
+        if (rc) return rc;
+        rc = batch_enqueue(batch, d_Q.as<double>(), st, nullptr);
         if (rc) return rc;
         const int nb = (int)std::min<long long>(kMsBlocks, (S + kMsThreads - 1) / kMsThreads);
         hipLaunchKernelGGL(big_broyden_pass1, dim3(nb), dim3(kMsThreads), 0, st, bro(), p.Broyden_norm_type,
@@ -435,11 +517,7 @@ class MultiScf {
             }
             if (adaptive_due(it)) {
                 // the adaptive correction samples this iteration's potential
-                hipLaunchKernelGGL(multi_potential_kernel, dim3(nblk_pot), dim3(kMsThreads), 0, st,
-                                   d_blocks.as<MsBlock>(), d_off.as<long long>(), d_kt.as<double>(), D,
-                                   bro().n_in, d_Uext.as<double>(), d_V.as<double>(), d_ptrU.as<double*>(),
-                                   d_ptrUc.as<double*>(), d_U.as<double>());
-                NEGF_HIP_TRY(hipGetLastError());
+                if ((rc = enqueue_potential())) break;
                 NEGF_HIP_TRY(hipStreamSynchronize(st));
             }
             bool rebuilt = false;
@@ -483,10 +561,10 @@ class MultiScf {
     negf_batch* batch = nullptr;
     std::vector<negf_tube*> tubes;
     std::vector<long long> off;
-    int T = 0, M = 0, D = 0, nblk_pot = 0;
+    int T = 0, M = 0, D = 0, ntiles = 0, nrows = 0;
     long long S = 0;
     negf_scf_params p{};
-    DevBuf d_kt, d_off, d_blocks, d_ptrU, d_ptrUc, d_Uext, d_V, d_U, d_Q, d_state, d_hist, d_part, d_coef, d_step;
+    DevBuf d_kt, d_off, d_tiles, d_rows, d_partial, d_ptrU, d_ptrUc, d_Uext, d_V, d_U, d_Q, d_state, d_hist, d_part, d_coef, d_step;
     HostBuf h_out;
     bool use_graph = true;
     long long gen = -1;
