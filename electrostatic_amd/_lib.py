@@ -60,6 +60,7 @@ class negf_tube_params(C.Structure):
         ("kT_window_around_singularity", C.c_double * 2),
         ("noneq_integration_pts_density", C.c_int * 8),
         ("flatband_dos_integration_pts", C.c_int),
+        ("flatband_dos_integration_limits", C.c_double * 2),
         ("segment_length", C.c_int),
     ]
 
@@ -90,6 +91,10 @@ SIGNATURES = {
     "negf_multi_scf_use_graph": (C.c_int, [vp, C.c_int]),
     "negf_multi_scf_bias_point": (C.c_int, [vp, vp, C.c_int, vp, vp, vp, vp, vp]),
     "negf_multi_scf_get_input_charge": (C.c_int, [vp, vp]),
+    "negf_tube_dos_complex": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
+    "negf_tube_dos_path": (C.c_int, [vp, C.c_int, C.c_int, vp, C.POINTER(C.c_int)]),
+    "negf_write_transport_char": (C.c_int, [C.c_char_p, C.c_int, vp, vp, vp, vp, vp, vp]),
+    "negf_write_ldos": (C.c_int, [C.c_char_p, C.c_int, vp, C.c_int, vp, vp]),
     "negf_rgf_diag_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp]),
     "negf_rgf_diag_accumulate_noalloc": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp,
                                                    vp]),

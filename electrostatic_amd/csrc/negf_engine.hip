@@ -313,7 +313,8 @@ struct RgfLayout {
     DecParams dec;
 };
 
-static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_chain_out, const int* tube_active)
+static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_chain_out, const int* tube_active,
+                             double* d_site_out = nullptr)
 {
     WLaunch lp;
     std::memset(&lp, 0, sizeof(lp));
@@ -337,6 +338,7 @@ static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_cha
     lp.dec_max_iter = y.dec.max_iter;
     lp.dec_rel_error = y.dec.rel_error;
     lp.tube_active = tube_active;
+    lp.site_out = d_site_out;
     return lp;
 }
 
@@ -391,7 +393,8 @@ static int launch_rgf_finish(const RgfLayout& y, const WLaunch& lp, double* d_ou
     return NEGF_OK;
 }
 
-int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st, const int* tube_active)
+int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st, const int* tube_active,
+                     double* d_site_out)
 {
     if (nitems_ == 0) {
         set_error("RgfPlan::enqueue: plan not built");
@@ -441,7 +444,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     y.red_N = d_red_N_.as<int>();
     y.red_dst_off = d_red_dst_off_.as<long long>();
     y.dec = dec_;
-    const WLaunch lp = launch_params(y, d_diagG, d_chain_out, tube_active);
+    const WLaunch lp = launch_params(y, d_diagG, d_chain_out, tube_active, d_site_out);
 
     if (dec_.use) {
         hipLaunchKernelGGL(sigma_dec_kernel, dim3((nchains_ + 255) / 256), dim3(256), 0, st, lp, y.chain_tube,

@@ -83,9 +83,10 @@ class RgfPlan {
     // (KIND_EQ/NONEQ); d_diagG: [chain][N]; d_chain_out: per-chain outputs
     // (integrand samples [slot][3], DOS/T pairs [slot][2]).
     // tube_active (device, indexed by RgfTube::ext, may be null): tubes with 0
-    // are skipped and keep their previous outputs
+    // are skipped and keep their previous outputs.  d_site_out (KIND_DOS,
+    // optional, [chain][N]): each site's Gamma_L |G_n0|^2 + Gamma_R |G_n,N-1|^2.
     int enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st,
-                const int* tube_active = nullptr);
+                const int* tube_active = nullptr, double* d_site_out = nullptr);
     // Read and clear the device status word (synchronises the stream).
     int check_status(hipStream_t st, int* status);
     int num_items() const { return nitems_; }

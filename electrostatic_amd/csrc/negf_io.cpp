@@ -13,6 +13,10 @@
 //                      (:4818-4840), appended one line per step.
 //   restart            Read_Table1D (:236-286): header line, N rows of
 //                      "position value", line count checked.
+//   transport_char.dat Write_Eql_Characteristics (:1931-1970): header, then
+//                      "%20g%20g%20g%35g%35g%35g" (setw, default precision).
+//   Ept_<e>.dat        Compute_DensityOfStates (:2778-2796) -> Write_Table1D,
+//                      header "PTD LDOS_r at E=" + std::to_string(E) ("%f").
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -196,5 +200,48 @@ extern "C" int negf_conductance(int nE, const double* E, const double* T, double
         sum += cond[i] * dE;
     }
     *total = sum;
+    return NEGF_OK;
+}
+
+extern "C" int negf_write_transport_char(const char* filename, int nE, const double* E, const double mu[2],
+                                         const double kT[2], const double* dos, const double* trans,
+                                         const double* cond)
+{
+    negf::clear_error();
+    if (!filename || nE <= 0 || !E || !mu || !kT || !dos || !trans || !cond) {
+        negf::set_error("invalid writer arguments");
+        return NEGF_EARG;
+    }
+    FILE* f = std::fopen(filename, "w");
+    if (!f) {
+        negf::set_error("cannot open %s for writing", filename);
+        return NEGF_EARG;
+    }
+    std::fprintf(f, " E_r, Fermi_Contact1, Fermi_Contact2, DOS_r, Transmission_r, Conductance_r \n");
+    for (int i = 0; i < nE; ++i) {
+        // FermiFunction (:1790-1795) of the real energy: 1 / (exp((E - mu)/kT) + 1)
+        const double f1 = 1.0 / (std::exp((E[i] - mu[0]) / kT[0]) + 1.0);
+        const double f2 = 1.0 / (std::exp((E[i] - mu[1]) / kT[1]) + 1.0);
+        std::fprintf(f, "%20g%20g%20g%35g%35g%35g\n", E[i], f1, f2, dos[i], trans[i], cond[i]);
+    }
+    std::fclose(f);
+    return NEGF_OK;
+}
+
+extern "C" int negf_write_ldos(const char* folder, int nE, const double* E, int N, const double* ptd_nm,
+                               const double* ldos)
+{
+    negf::clear_error();
+    if (!folder || nE <= 0 || N <= 0 || !E || !ptd_nm || !ldos) {
+        negf::set_error("invalid writer arguments");
+        return NEGF_EARG;
+    }
+    for (int e = 0; e < nE; ++e) {
+        char head[128];
+        std::snprintf(head, sizeof(head), "PTD LDOS_r at E=%f", E[e]);
+        const std::string name = std::string(folder) + "/Ept_" + std::to_string(e) + ".dat";
+        const int rc = write_table(name, head, N, ptd_nm, ldos + (size_t)e * N);
+        if (rc) return rc;
+    }
     return NEGF_OK;
 }

@@ -44,6 +44,8 @@ class Tube {
     int take_root_flag();
     int total_integration_pts() const;
     int dos(int nE, const double* E_real, double* dos_out, double* trans_out);
+    int dos_complex(int nE, const std::complex<double>* E, double* dos_out, double* trans_out, double* ldos);
+    int dos_path(int which, std::vector<std::complex<double>>& E) const;
     // Terminal currents (A) over the current non-equilibrium path.
     int current(double I[2]);
 

@@ -665,14 +665,27 @@ int Tube::total_integration_pts() const
 // T = sum_m degen_m Gamma_L Gamma_R |G_0,N-1|^2.
 int Tube::dos(int nE, const double* E_real, double* dos_out, double* trans_out)
 {
+    std::vector<cplx> E(nE);
+    for (int e = 0; e < nE; ++e) E[e] = cplx(E_real[e], p.E_zPlus_imag);
+    return dos_complex(nE, E.data(), dos_out, trans_out, nullptr);
+}
+
+// Compute_DensityOfStates (NEGF_Common.cpp:2390-2847) at the energies E:
+// per energy the trace of the spectral function and the transmission
+// (DOS = spin_degen sum_n LDOS_n / atoms per unit cell, T = Gamma_L Gamma_R
+// |G_0,N-1|^2, summed over the modes with their degeneracies), and
+// optionally LDOS_n = sum_modes degen A_nn / (2 pi) per site ([nE][N],
+// the values the reference writes to Ept_<e>.dat, :2778-2796).
+int Tube::dos_complex(int nE, const cplx* E, double* dos_out, double* trans_out, double* ldos)
+{
     const int nm = (int)beta.size();
     std::vector<ChainDesc> ch;
     for (int m = 0; m < nm; ++m)
         for (int e = 0; e < nE; ++e) {
             ChainDesc d;
             std::memset(&d, 0, sizeof(d));
-            d.Er = E_real[e];
-            d.Ei = p.E_zPlus_imag;
+            d.Er = E[e].real();
+            d.Ei = E[e].imag();
             d.t_even = beta[m];
             d.degen = degen[m];
             d.out_slot = m * nE + e;
@@ -681,12 +694,21 @@ int Tube::dos(int nE, const double* E_real, double* dos_out, double* trans_out)
     int rc;
     if ((rc = upload_chains(dos_chains, ch))) return rc;
     if ((rc = dos_pairs.ensure(sizeof(double) * 2 * ch.size()))) return rc;
+    DevBuf d_site;
+    if (ldos && (rc = d_site.ensure(sizeof(double) * ch.size() * (size_t)N))) return rc;
     RgfPlan plan;
     std::vector<RgfTube> tubes{rgf_tube(d_U.as<double>(), d_Uc.as<double>(), (int)ch.size())};
     if ((rc = plan.build(KIND_DOS, false, tubes, dos_chains.as<ChainDesc>(), (int)ch.size(), dec()))) return rc;
-    if ((rc = plan.enqueue(nullptr, nullptr, dos_pairs.as<double>(), stream))) return rc;
+    if ((rc = plan.enqueue(nullptr, nullptr, dos_pairs.as<double>(), stream, nullptr,
+                           ldos ? d_site.as<double>() : nullptr)))
+        return rc;
     std::vector<double> h(2 * ch.size());
     NEGF_HIP_TRY(hipMemcpyAsync(h.data(), dos_pairs.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+    std::vector<double> hs;
+    if (ldos) {
+        hs.resize(ch.size() * (size_t)N);
+        NEGF_HIP_TRY(hipMemcpyAsync(hs.data(), d_site.p, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, stream));
+    }
     int status = 0;
     if ((rc = plan.check_status(stream, &status))) return rc;
     root_flag |= status & 1;
@@ -698,7 +720,43 @@ int Tube::dos(int nE, const double* E_real, double* dos_out, double* trans_out)
         }
         dos_out[e] = spin_degen * tr / atoms_per_unitcell;
         trans_out[e] = tt;
+        if (ldos)
+            for (int n = 0; n < N; ++n) {
+                double v = 0.0;
+                for (int m = 0; m < nm; ++m) v += degen[m] * hs[((size_t)m * nE + e) * N + n] / (2.0 * kPi);
+                ldos[(size_t)e * N + n] = v;
+            }
     }
+    return NEGF_OK;
+}
+
+// Energies of the DOS paths (NEGF_Common.cpp:1807-1815, 2210-2249):
+// which = 0, the flat-band path of Initialize_NEGF: a Gauss-Legendre line
+// from flatband_dos_integration_limits[0] + i eta to [1] + i eta with
+// flatband_dos_integration_pts points; which = 1, the per-step path of
+// Update_IntegrationPaths: the non-equilibrium path when the contacts differ,
+// else a line from Re(E_eta) (imaginary part 0, as the reference passes the
+// real part alone) to E_rightmost with flatband_dos_integration_pts points.
+int Tube::dos_path(int which, std::vector<cplx>& E) const
+{
+    const int n = p.flatband_dos_integration_pts;
+    E.clear();
+    if (which == 1 && L.noneq) {
+        E = neq_path.E;
+        return NEGF_OK;
+    }
+    if (n <= 0) {
+        set_error("flatband_dos_integration_pts must be positive");
+        return NEGF_EARG;
+    }
+    std::vector<double> Ev(2 * n), w(n), mul(2 * n);
+    if (which == 0)
+        gl_path(p.flatband_dos_integration_limits[0], p.E_zPlus_imag, p.flatband_dos_integration_limits[1],
+                p.E_zPlus_imag, n, 0, Ev.data(), w.data(), mul.data());
+    else
+        gl_path(L.E_eta.real(), 0.0, L.E_rightmost.real(), L.E_rightmost.imag(), n, 0, Ev.data(), w.data(),
+                mul.data());
+    for (int e = 0; e < n; ++e) E.push_back(cplx(Ev[2 * e], Ev[2 * e + 1]));
     return NEGF_OK;
 }
 
@@ -781,6 +839,8 @@ void negf_tube_params_default(negf_tube_params* p)
     p->kT_window_around_singularity[0] = 1.0;
     p->kT_window_around_singularity[1] = 1.0;
     p->flatband_dos_integration_pts = 100;
+    p->flatband_dos_integration_limits[0] = -1.0;  // NEGF_Common.H:417
+    p->flatband_dos_integration_limits[1] = 1.0;
 }
 
 int negf_tube_create(const negf_tube_params* p, negf_tube** out)
@@ -1017,6 +1077,42 @@ int negf_tube_dos(negf_tube* t, int nE, const double* E_real, double* dos, doubl
         return NEGF_EARG;
     }
     return t->t.dos(nE, E_real, dos, trans);
+}
+
+int negf_tube_dos_complex(negf_tube* t, int nE, const negf_complex* E, double* dos, double* trans, double* ldos)
+{
+    clear_error();
+    if (!t || nE <= 0 || !E || !dos || !trans) {
+        set_error("invalid DOS arguments");
+        return NEGF_EARG;
+    }
+    std::vector<cplx> Ec(nE);
+    for (int e = 0; e < nE; ++e) Ec[e] = cplx(E[e].re, E[e].im);
+    return t->t.dos_complex(nE, Ec.data(), dos, trans, ldos);
+}
+
+int negf_tube_dos_path(negf_tube* t, int which, int max_pts, negf_complex* E, int* n)
+{
+    clear_error();
+    if (!t || !n || (which != 0 && which != 1)) {
+        set_error("invalid DOS path arguments");
+        return NEGF_EARG;
+    }
+    std::vector<cplx> Ev;
+    int rc = t->t.dos_path(which, Ev);
+    if (rc) return rc;
+    *n = (int)Ev.size();
+    if (E) {
+        if ((int)Ev.size() > max_pts) {
+            set_error("DOS path has %d points, buffer holds %d", (int)Ev.size(), max_pts);
+            return NEGF_EARG;
+        }
+        for (size_t e = 0; e < Ev.size(); ++e) {
+            E[e].re = Ev[e].real();
+            E[e].im = Ev[e].imag();
+        }
+    }
+    return NEGF_OK;
 }
 
 }  // extern "C"

@@ -71,6 +71,7 @@ struct WLaunch {
     double* Dd;               // [chain][3]: D (re, im) + exponent
     double* part;             // partial charge rows [group][N]
     double2* diagG;           // optional [chain][N]
+    double* site_out;         // KIND_DOS, optional [chain][N]: the site's spectral weight
     double* chain_out;        // per-chain outputs
     double* dos_part;         // [item][64] per-segment DOS traces
     const int* sblk_tube;     // chain groups: tube
@@ -786,7 +787,7 @@ struct SubCtx {
 template <int K, int KIND, bool WRITE_G, bool FULL, bool SAMPLE>
 __device__ __forceinline__ void site_loop(const Coefs& k, const double (&Us)[K], const double2 (&vr)[K], double2& ux,
                                           double2& uy, const SubCtx& c, int p0, int len, bool valid,
-                                          double2* Grow, double* red, int lane, double& dos_acc,
+                                          double2* Grow, double* Srow, double* red, int lane, double& dos_acc,
                                           const WTube& tube, const ChainDesc& ch, double* chain_out)
 {
 #pragma unroll
@@ -808,7 +809,9 @@ __device__ __forceinline__ void site_loop(const Coefs& k, const double (&Us)[K],
                             chain_out[(long long)ch.out_slot * 3 + w] = ch.iL * c.aL * av + ch.iR * c.aR * au;
                 }
             } else {
-                dos_acc += c.aL * cabs2(vr[i]) + c.aR * cabs2(ux);
+                const double a = c.aL * cabs2(vr[i]) + c.aR * cabs2(ux);
+                dos_acc += a;
+                if (Srow && valid) Srow[p0 + i] = a;
             }
             if (i & 1)
                 fwd_step<1>(k, Us[i], ux, uy);
@@ -887,6 +890,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
     int eW = 0;                           // and its exponent
     double dos_acc = 0.0;
     double2* Grow = WRITE_G ? (lp.diagG + (long long)ic.chain * N) : nullptr;
+    double* Srow = (KIND == KIND_DOS && lp.site_out) ? (lp.site_out + (long long)ic.chain * N) : nullptr;
     for (int j = 0; j < nsub; ++j) {
         const int p0 = lo + j * K;
         const int len = min(K, hi - p0);
@@ -956,13 +960,13 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
             for (int w = 0; w < 3; ++w) samp |= (tube.sample[w] >= p0 && tube.sample[w] < p0 + len);
         }
         if (samp)
-            site_loop<K, KIND, WRITE_G, false, true>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, red, lane,
+            site_loop<K, KIND, WRITE_G, false, true>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, Srow, red, lane,
                                                      dos_acc, tube, ch, lp.chain_out);
         else if (len == K)
-            site_loop<K, KIND, WRITE_G, true, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, red, lane,
+            site_loop<K, KIND, WRITE_G, true, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, Srow, red, lane,
                                                      dos_acc, tube, ch, lp.chain_out);
         else
-            site_loop<K, KIND, WRITE_G, false, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, red, lane,
+            site_loop<K, KIND, WRITE_G, false, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, Srow, red, lane,
                                                       dos_acc, tube, ch, lp.chain_out);
         if (KIND != KIND_DOS) {
             double r = lane_sum_transpose<K>(red, lane);

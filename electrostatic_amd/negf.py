@@ -153,6 +153,24 @@ class Tube:
         _lib.check(self.lib.negf_tube_dos(self.h, E.size, _dp(E), _dp(d), _dp(t)))
         return d, t
 
+    def dos_complex(self, E, want_ldos=False):
+        """DOS, transmission (and LDOS [nE][N]) at complex energies."""
+        Ec = np.ascontiguousarray(E, np.complex128)
+        d = np.zeros(Ec.size)
+        t = np.zeros(Ec.size)
+        ld = np.zeros((Ec.size, self.N)) if want_ldos else None
+        _lib.check(self.lib.negf_tube_dos_complex(self.h, Ec.size, Ec.ctypes.data, _dp(d), _dp(t),
+                                                  _dp(ld) if want_ldos else None))
+        return (d, t, ld) if want_ldos else (d, t)
+
+    def dos_path(self, which):
+        """Energies of the reference's DOS path: 0 flat-band, 1 per step."""
+        n = C.c_int()
+        _lib.check(self.lib.negf_tube_dos_path(self.h, int(which), 0, None, C.byref(n)))
+        E = np.zeros(n.value, np.complex128)
+        _lib.check(self.lib.negf_tube_dos_path(self.h, int(which), n.value, E.ctypes.data, C.byref(n)))
+        return E
+
 
 class Batch:
     """Many tubes solved in one launch sequence (negf_batch_*)."""

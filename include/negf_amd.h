@@ -169,6 +169,7 @@ typedef struct {
                                     /* with adaptive limits; all 0: not    */
                                     /* given, set from the first correction */
     int flatband_dos_integration_pts;
+    double flatband_dos_integration_limits[2];  /* eV (NEGF_Common.H:417)  */
     int segment_length;             /* RGF segment length (0: default); no */
                                     /* reference counterpart (performance)  */
 } negf_tube_params;
@@ -235,6 +236,20 @@ int negf_tube_current(negf_tube *t, double *I);
  * T = sum_modes degen * Gamma_L Gamma_R |G_0,N-1|^2 (host arrays). */
 int negf_tube_dos(negf_tube *t, int nE, const double *E_real, double *dos,
                   double *trans);
+/* The same at complex energies E (host array of nE), with the per-site
+ * LDOS_n = sum_modes degen A_nn / (2 pi) (:2733-2742) into ldos ([nE][N],
+ * optional): the values of the reference's Ept_<e>.dat files. */
+int negf_tube_dos_complex(negf_tube *t, int nE, const negf_complex *E,
+                          double *dos, double *trans, double *ldos);
+/* Energies of the reference's DOS paths: which = 0, the flat-band path of
+ * Initialize_NEGF (NEGF_Common.cpp:1807-1815): flatband_dos_integration_pts
+ * Gauss-Legendre points from flatband_dos_integration_limits[0] + i eta to
+ * [1] + i eta; which = 1, the per-step path of Update_IntegrationPaths
+ * (:2210-2249): the non-equilibrium path when the contacts differ, else
+ * flatband_dos_integration_pts points from Re(E_eta) (imaginary part 0) to
+ * E_rightmost.  *n receives the count; E (optional, max_pts) the energies. */
+int negf_tube_dos_path(negf_tube *t, int which, int max_pts, negf_complex *E,
+                       int *n);
 
 /* ------------------------------------------------------------------------
  * Batched solve of many tubes (no reference counterpart: the reference loops
@@ -519,6 +534,18 @@ int negf_write_current_line(const char *filename, int step, double Vds,
 /* Restart: read a charge distribution written as <prefix>_Qout.dat
  * (Read_Table1D NEGF_Common.cpp:236-286; header + N rows required). */
 int negf_read_charge_distribution(const char *filename, int N, double *Q);
+/* transport_char.dat (Write_Eql_Characteristics NEGF_Common.cpp:1931-1970):
+ * header line, then per energy "E f_1 f_2 DOS T G" at setw 20/20/20/35/35/35
+ * and the stream's default precision, f_k = 1/(exp((E - mu_k)/kT_k) + 1). */
+int negf_write_transport_char(const char *filename, int nE, const double *E_real,
+                              const double mu[2], const double kT[2],
+                              const double *dos, const double *trans,
+                              const double *cond);
+/* <folder>/Ept_<e>.dat for e < nE (NEGF_Common.cpp:2778-2796 via
+ * Write_Table1D :4223-4254): header "PTD LDOS_r at E=<E_e, %f>", then N rows
+ * "position LDOS" at setprecision(15) setw(35).  ldos: [nE][N]. */
+int negf_write_ldos(const char *folder, int nE, const double *E_real, int N,
+                    const double *ptd_nm, const double *ldos);
 /* Compute_Conductance (NEGF_Common.cpp:1873-1928): per-energy conductance
  * -T dF/dE (units of 2q^2/h, finite differences on the given grid) and its
  * sum times the reference's step E_0 - E_1. */

@@ -175,6 +175,12 @@ void orc_write_charge_components(const char *filename, int N,
                                  const double *neq, const double *r0,
                                  const double *Q);
 void orc_write_current_header(const char *filename);
+void orc_write_transport_char(const char *filename, int nE, const double *E,
+                              const double *mu, const double *kT,
+                              const double *dos, const double *tran,
+                              const double *cond);
+void orc_write_ldos(const char *folder, int nE, const double *E, int N,
+                    const double *ptd, const double *ldos);
 void orc_write_current_line(const char *filename, int step, double Vds,
                             double Vgs, const double *I, double avg_pts,
                             int iters, double conductance);

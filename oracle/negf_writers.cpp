@@ -9,7 +9,11 @@
  *   _Qin.dat   Write_InputInducedCharge :3081-3092 -> Write_Table1D
  *   _chargeComp.dat  Write_ChargeComponents :2999-3023
  *   I.dat      Define_FileHeaderForCurrent :73-82, Write_Current :4818-4840
+ *   transport_char.dat  Write_Eql_Characteristics :1931-1970
+ *   Ept_<e>.dat  Compute_DensityOfStates :2778-2796 -> Write_Table1D
  */
+#include <cmath>
+#include <complex>
 #include <fstream>
 #include <iomanip>
 #include <string>
@@ -88,4 +92,39 @@ extern "C" void orc_write_current_line(const char *filename, int step,
     for (int k = 0; k < 2; ++k) f << std::setw(20) << I[k];
     f << std::setw(10) << avg_pts << std::setw(10) << iters << std::setw(20)
       << conductance << "\n";
+}
+
+/* Write_Eql_Characteristics NEGF_Common.cpp:1931-1970 (the energies arrive
+ * real, E_total_vec :2806-2815); FermiFunction :1790-1795 on complex E - mu. */
+extern "C" void orc_write_transport_char(const char *filename, int nE,
+                                         const double *E, const double *mu,
+                                         const double *kT, const double *dos,
+                                         const double *tran,
+                                         const double *cond)
+{
+    std::ofstream outfile(filename);
+    outfile << " E_r, Fermi_Contact1, Fermi_Contact2, DOS_r, Transmission_r, "
+               "Conductance_r \n";
+    auto fermi = [](std::complex<double> z, double kt) {
+        return (1. / (std::exp(z / kt) + 1.)).real();
+    };
+    for (int i = 0; i < nE; ++i)
+        outfile << std::setw(20) << E[i] << std::setw(20)
+                << fermi(std::complex<double>(E[i], 0.) - mu[0], kT[0])
+                << std::setw(20)
+                << fermi(std::complex<double>(E[i], 0.) - mu[1], kT[1])
+                << std::setw(35) << dos[i] << std::setw(35) << tran[i]
+                << std::setw(35) << cond[i] << "\n";
+}
+
+/* Ept_<e>.dat, NEGF_Common.cpp:2778-2796: Write_Table1D of the site
+ * positions and the LDOS at energy e, header "PTD LDOS_r at E=" +
+ * std::to_string(E.real()). */
+extern "C" void orc_write_ldos(const char *folder, int nE, const double *E,
+                               int N, const double *ptd, const double *ldos)
+{
+    for (int e = 0; e < nE; ++e)
+        table(std::string(folder) + "/Ept_" + std::to_string(e) + ".dat",
+              "PTD LDOS_r at E=" + std::to_string(E[e]), N, ptd,
+              ldos + (size_t)e * N);
 }

@@ -465,6 +465,18 @@ def scf_bias_point(t, Rho0, K, V_gate, broyden, fraction=0.1, max_norm=1e-6,
     return U, Q, it.value, fn.value, st
 
 
+def write_transport_char(filename, E, mu, kT, dos, tran, cond):
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (E, mu, kT, dos, tran, cond)]
+    lib().orc_write_transport_char(filename.encode(), arrs[0].size, *[_dp(a) for a in arrs])
+
+
+def write_ldos(folder, E, ptd, ldos):
+    E = np.ascontiguousarray(E, np.float64)
+    ptd = np.ascontiguousarray(ptd, np.float64)
+    ldos = np.ascontiguousarray(ldos, np.float64)
+    lib().orc_write_ldos(folder.encode(), E.size, _dp(E), ptd.size, _dp(ptd), _dp(ldos))
+
+
 def write_step_files(prefix, ptd, Q=None, U=None, norm=None):
     ptd = np.ascontiguousarray(ptd, np.float64)
     arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (Q, U, norm)]

@@ -161,3 +161,41 @@ def test_conductance_of_a_perfect_channel():
     # two conducting modes double it
     _, tot2 = negf.conductance(E, 2 * np.ones_like(E), -1.0, kT)
     assert abs(tot2 - 2 * tot) < 1e-14
+
+
+def test_transport_char_and_ldos_writers_match_oracle(tmp_path):
+    """transport_char.dat (Write_Eql_Characteristics NEGF_Common.cpp:1931-1970)
+    and Ept_<e>.dat (:2778-2796 via Write_Table1D) byte for byte against the
+    oracle's iostream restatement, including Fermi factors that underflow."""
+    lib = _lib.load()
+    rng = np.random.default_rng(7)
+    nE, N = 37, 23
+    E = np.sort(rng.uniform(-3.0, 3.0, nE))
+    E[0] = -40.0           # exp(-39/kT) underflows to 0: f = 1
+    E[-1] = 45.0           # exp overflows: f = 0
+    mu = np.array([-1.0, -1.2])
+    kT = np.array([0.025680, 0.02])
+    dos = rng.uniform(0, 5, nE) * 10.0 ** rng.integers(-12, 3, nE)
+    tr = rng.uniform(0, 2, nE)
+    cond = rng.normal(size=nE) * 1e-3
+    a = str(tmp_path / "a.dat")
+    b = str(tmp_path / "b.dat")
+    rc = lib.negf_write_transport_char(a.encode(), nE, _ptr(E), _ptr(mu), _ptr(kT), _ptr(dos), _ptr(tr), _ptr(cond))
+    assert rc == 0
+    O.write_transport_char(b, E, mu, kT, dos, tr, cond)
+    assert open(a, "rb").read() == open(b, "rb").read()
+    ptd = np.linspace(-2.0, 2.0, N)
+    ldos = rng.uniform(0, 1, (nE, N)) * 10.0 ** rng.integers(-8, 2, (nE, N))
+    (tmp_path / "g").mkdir()
+    (tmp_path / "o").mkdir()
+    rc = lib.negf_write_ldos(str(tmp_path / "g").encode(), nE, _ptr(E), N, _ptr(ptd), _ptr(ldos))
+    assert rc == 0
+    O.write_ldos(str(tmp_path / "o"), E, ptd, ldos)
+    for e in range(nE):
+        name = f"Ept_{e}.dat"
+        assert (tmp_path / "g" / name).read_bytes() == (tmp_path / "o" / name).read_bytes()
+
+
+def _ptr(a):
+    import ctypes as C
+    return a.ctypes.data_as(C.c_void_p)

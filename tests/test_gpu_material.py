@@ -267,3 +267,48 @@ def test_current_matches_oracle():
     Io, st = O.current(ot, U, Uc)
     assert np.abs(I - Io).max() < 1e-10 * np.abs(Io).max()
     assert I[0] > 0 and abs(I[0] + I[1]) < 1e-12 * abs(I[0])
+
+
+@pytest.mark.parametrize("mu", [(-1.0, -1.0), (-1.0, -1.15)])
+def test_dos_paths_and_ldos_match_oracle(mu):
+    """The reference's two DOS paths (NEGF_Common.cpp:1807-1815, 2210-2249)
+    and the DOS, transmission and per-site LDOS along them (:2640-2796),
+    against the path formulas (numpy Gauss-Legendre) and the oracle."""
+    eta = 1e-5
+    tube, ot = pair(num_unitcells=15, num_modes=2, mu=mu, eta=eta, flatband_dos_integration_pts=40,
+                    flatband_dos_integration_limits=(-2.0, 0.5))
+    x, _ = np.polynomial.legendre.leggauss(40)
+    # flat-band path, right after creation (U = 0): line -2 + i eta -> 0.5 + i eta
+    E0 = tube.dos_path(0)
+    assert np.abs(E0 - (1.25 * x - 0.75 + 1j * eta)).max() < 1e-12
+    d, t, ld = tube.dos_complex(E0, want_ldos=True)
+    do, to, ldo, st = O.dos_transmission(ot, np.zeros(tube.N), np.zeros(2), E0, 68, want_ldos=True)
+    assert st == 0
+    assert rel(d, do) < 1e-10 and rel(ld, ldo) < 1e-10
+    assert np.abs(t - to).max() < 1e-10 * max(1.0, np.abs(to).max())
+    # the DOS trace is the sum of the LDOS (spin 2 over 68 atoms per unit cell)
+    assert rel(d, 2 * ld.sum(axis=1) / 68) < 1e-12
+    # per-step path after a solve at a potential
+    U = gaussian_bumps(tube.N, seed=3, amp=0.2)
+    Uc = (U[0], U[-1])
+    tube.set_potential(U, Uc)
+    tube.solve(1)
+    E1 = tube.dos_path(1)
+    lim = tube.limits()
+    if mu[0] != mu[1]:
+        # the non-equilibrium path itself: E_cr -> E_rightmost on the real
+        # axis (+ i eta), 200 points
+        xn, _ = np.polynomial.legendre.leggauss(200)
+        a, b = lim["E_contour_right"], lim["E_rightmost"]
+        assert E1.size == 200
+        assert np.abs(E1 - ((b - a) / 2 * xn + (b + a) / 2 + 1j * eta)).max() < 1e-12
+    else:
+        # Re(E_eta) with imaginary part 0 -> E_rightmost (+ i eta)
+        a, b = lim["E_eta"], complex(lim["E_rightmost"], eta)
+        want = (b - a) / 2 * x + (b + a) / 2
+        assert np.abs(E1 - want).max() < 1e-12
+    d, t, ld = tube.dos_complex(E1, want_ldos=True)
+    do, to, ldo, st = O.dos_transmission(ot, U, Uc, E1, 68, want_ldos=True)
+    assert rel(d, do) < 1e-10 and rel(ld, ldo) < 1e-10
+    assert np.abs(t - to).max() < 1e-10 * max(1.0, np.abs(to).max())
+    tube.close()
