@@ -95,6 +95,8 @@ SIGNATURES = {
     "negf_tube_dos_path": (C.c_int, [vp, C.c_int, C.c_int, vp, C.POINTER(C.c_int)]),
     "negf_write_transport_char": (C.c_int, [C.c_char_p, C.c_int, vp, vp, vp, vp, vp, vp]),
     "negf_write_ldos": (C.c_int, [C.c_char_p, C.c_int, vp, C.c_int, vp, vp]),
+    "negf_batch_autotune_cached": (C.c_int, [vp, vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int)]),
     "negf_rgf_diag_accumulate": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp, vp]),
     "negf_rgf_diag_accumulate_noalloc": (C.c_int, [C.POINTER(negf_chain_params), vp, C.c_int, vp, vp, vp, vp,
                                                    vp]),

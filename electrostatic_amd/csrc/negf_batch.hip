@@ -12,6 +12,9 @@
 // length by timing the solve.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <string>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -253,6 +256,73 @@ class Batch {
         return NEGF_OK;
     }
 
+    // Workload key of the tuning cache: device (architecture and compute
+    // units) and, per tube, its sites and chain counts (FNV-1a hash).
+    int tune_key(std::string& key)
+    {
+        int dev = 0;
+        NEGF_HIP_TRY(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        NEGF_HIP_TRY(hipGetDeviceProperties(&prop, dev));
+        unsigned long long h = 1469598103934665603ull;
+        auto mix = [&h](long long v) {
+            for (int b = 0; b < 8; ++b) {
+                h ^= (unsigned long long)((v >> (8 * b)) & 0xff);
+                h *= 1099511628211ull;
+            }
+        };
+        for (Tube* t : tubes) {
+            mix(t->N);
+            mix((long long)t->eq_host.size());
+            mix((long long)t->neq_host.size());
+        }
+        char buf[256];
+        std::snprintf(buf, sizeof(buf), "rgf-batch-v1 %s cu%d tubes%d %016llx", prop.gcnArchName,
+                      prop.multiProcessorCount, (int)tubes.size(), h);
+        for (char* c = buf; *c; ++c)
+            if (*c == ':') *c = '_';
+        key = buf;
+        return NEGF_OK;
+    }
+    // autotune() with a persistent cache: a text file of "key L ms" lines.
+    // A hit applies the stored segment length without timing; a miss tunes
+    // and appends the result.
+    int autotune_cached(double* dQ, const char* path, int* best_L, double* best_ms, int* hit)
+    {
+        std::string key;
+        int rc;
+        if ((rc = tune_key(key))) return rc;
+        *hit = 0;
+        if (FILE* f = std::fopen(path, "r")) {
+            char line[512];
+            const size_t kl = key.size();
+            while (std::fgets(line, sizeof(line), f)) {
+                int L = 0;
+                double ms = 0.0;
+                if (!std::strncmp(line, key.c_str(), kl) && line[kl] == ' ' &&
+                    std::sscanf(line + kl, "%d %lf", &L, &ms) == 2 && L > 0) {
+                    *hit = 1;  // the last matching line wins
+                    *best_L = L;
+                    *best_ms = ms;
+                }
+            }
+            std::fclose(f);
+        }
+        if (*hit) {
+            seg_len = *best_L;
+            return build();
+        }
+        if ((rc = autotune(dQ, best_L, best_ms))) return rc;
+        FILE* f = std::fopen(path, "a");
+        if (!f) {
+            set_error("cannot append to the tuning cache %s", path);
+            return NEGF_EARG;
+        }
+        std::fprintf(f, "%s %d %.6f\n", key.c_str(), *best_L, *best_ms);
+        std::fclose(f);
+        return NEGF_OK;
+    }
+
     // Read the plans' device status words into the sticky root flag
     // (synchronises the batch stream).
     int poll_status()
@@ -415,6 +485,17 @@ int negf_batch_autotune(negf_batch* b, double* d_Q, int* segment_length, double*
         return NEGF_EARG;
     }
     return b->b.autotune(d_Q, segment_length, ms);
+}
+
+int negf_batch_autotune_cached(negf_batch* b, double* d_Q, const char* cache_path, int* segment_length, double* ms,
+                               int* from_cache)
+{
+    clear_error();
+    if (!b || !d_Q || !cache_path || !segment_length || !ms || !from_cache) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    return b->b.autotune_cached(d_Q, cache_path, segment_length, ms, from_cache);
 }
 
 int negf_batch_enable_timing(negf_batch* b, int on)

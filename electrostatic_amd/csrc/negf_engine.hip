@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 
@@ -150,6 +151,16 @@ int RgfPlan::kernel_time(double* total_ms, int* launches)
     return NEGF_OK;
 }
 
+// Tubes of FUSED_MIN_SEGS..FUSED_SEGS segments take the fused transfer and
+// scan launch (NEGF_RGF_FUSED=0 in the environment turns it off, which leaves
+// the results the same up to rounding).
+static int fused_for(int P)
+{
+    const char* e = std::getenv("NEGF_RGF_FUSED");
+    if (e && e[0] == '0') return 0;
+    return P >= FUSED_MIN_SEGS && P <= FUSED_SEGS ? 1 : 0;
+}
+
 // Kernel-side descriptor of one tube whose items start at item0 and whose
 // partial rows start at part_off.
 static WTube make_wtube(const RgfTube& tb, int P, int L, int item0, long long part_off)
@@ -171,6 +182,7 @@ static WTube make_wtube(const RgfTube& tb, int P, int L, int item0, long long pa
     w.part_off = part_off;
     for (int q = 0; q < 3; ++q) w.sample[q] = tb.sample[q];
     w.ext = tb.ext;
+    w.fused = fused_for(P);
     return w;
 }
 
@@ -211,6 +223,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     max_N_ = 0;
     max_P_ = 0;
     max_groups_ = 0;
+    any_fused_ = any_unfused_ = false;
     for (int t = 0; t < ntubes_; ++t) {
         const RgfTube& tb = tubes[t];
         if (tb.N < 2 || !tb.dU || tb.nchain <= 0 || tb.t_odd <= 0 || tb.chain0 < 0 ||
@@ -226,6 +239,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         }
         WTube& w = wt[t];
         w = make_wtube(tb, P, L, items, poff);
+        (w.fused ? any_fused_ : any_unfused_) = true;
         const int G = (tb.nchain + 63) / 64;
         for (int i = 0; i < G * P; ++i) {
             item_tube.push_back(t);
@@ -263,6 +277,8 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     if ((rc = d_part_.ensure(std::max<long long>(1, poff) * sizeof(double)))) return rc;
     if (kind == KIND_DOS && (rc = d_dos_part_.ensure((size_t)items * 64 * sizeof(double)))) return rc;
     if ((rc = upload(d_tubes_, wt))) return rc;
+    tube0_.assign(reinterpret_cast<const unsigned char*>(wt.data()),
+                  reinterpret_cast<const unsigned char*>(wt.data()) + sizeof(WTube));
     if ((rc = upload(d_item_tube_, item_tube))) return rc;
     {
         std::vector<int> all;
@@ -311,6 +327,8 @@ struct RgfLayout {
     const int* red_N = nullptr;
     const long long* red_dst_off = nullptr;
     DecParams dec;
+    bool any_fused = false, any_unfused = true;  // tubes with and without the fused transfer and scan
+    const WTube* tube1 = nullptr;                // host copy of the descriptor when ntubes == 1
 };
 
 static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_chain_out, const int* tube_active,
@@ -330,6 +348,8 @@ static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_cha
     lp.chain_out = d_chain_out;
     lp.dos_part = y.dos_part;
     lp.sblk_tube = y.sblk_tube;
+    lp.single = y.ntubes == 1 && y.tube1 ? 1 : 0;
+    if (lp.single) lp.tube1 = *y.tube1;
     lp.sblk_chain = y.sblk_chain;
     lp.status = y.status;
     lp.kind = y.kind;
@@ -345,8 +365,12 @@ static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_cha
 // Transfer, scan and sweeps of one layout on a stream.
 static int launch_rgf_kernels(const RgfLayout& y, const WLaunch& lp, hipStream_t st)
 {
-    hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((y.nitems + 3) / 4), dim3(256), 0, st, lp);
-    hipLaunchKernelGGL(scan_kernel, dim3(2 * y.nsblk), dim3(64 * SCAN_WAVES), 0, st, lp, y.nsblk);
+    if (y.any_unfused) {
+        hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((y.nitems + 3) / 4), dim3(256), 0, st, lp);
+        hipLaunchKernelGGL(scan_kernel, dim3(2 * y.nsblk), dim3(64 * SCAN_WAVES), 0, st, lp, y.nsblk);
+    }
+    if (y.any_fused)
+        hipLaunchKernelGGL(transfer_scan_kernel, dim3(2 * y.nsblk), dim3(64 * FUSED_WAVES), 0, st, lp, y.nsblk);
     for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
         if (y.fast_n[ns] == 0) continue;
         WLaunch lf = lp;
@@ -444,6 +468,9 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     y.red_N = d_red_N_.as<int>();
     y.red_dst_off = d_red_dst_off_.as<long long>();
     y.dec = dec_;
+    y.any_fused = any_fused_;
+    y.any_unfused = any_unfused_;
+    if (ntubes_ == 1 && tube0_.size() == sizeof(WTube)) y.tube1 = reinterpret_cast<const WTube*>(tube0_.data());
     const WLaunch lp = launch_params(y, d_diagG, d_chain_out, tube_active, d_site_out);
 
     if (dec_.use) {
@@ -781,6 +808,9 @@ int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d
         y.slow_n = (int)items;
     }
     const WTube tb = make_wtube(one_tube(p, d_U, nE), P, L, 0, 0);
+    y.any_fused = tb.fused != 0;
+    y.any_unfused = !y.any_fused;
+    y.tube1 = &tb;
 
     hipLaunchKernelGGL(pack_eq_chains, dim3((nE + 255) / 256), dim3(256), 0, st, nE, (const double2*)d_E,
                        (const double2*)d_coef, p->t_even, y.chains);

@@ -59,6 +59,7 @@ struct WTube {
     int sample[3];        // non-equilibrium integrand sample sites (-1: none)
     int ext;              // index of the tube in its caller (the active mask)
     const double* Ucp;    // device contact potentials (2 values), or null: Uc0/Uc1
+    int fused;            // transfer matrices and scan in one launch (transfer_scan_kernel)
 };
 
 struct WLaunch {
@@ -82,7 +83,22 @@ struct WLaunch {
     int nlist;
     const int* tube_active;   // optional [ext]: 0 skips the tube's work (its outputs stay as they were)
     int dec_use, dec_layers, dec_max_iter, dec_rel_error;  // lead surface GF by decimation (else closed form)
+    int single;               // one tube: its descriptor is tube1 (a kernel argument, no table load)
+    WTube tube1;
 };
+
+// Tube of a chain group or item: from the kernel argument when the launch has
+// one tube (saves a dependent load at the start of every wavefront).
+__device__ __forceinline__ int group_tube(const WLaunch& lp, int gi, int& g)
+{
+    if (lp.single) {
+        g = gi;
+        return 0;
+    }
+    g = lp.sblk_chain[gi];
+    return lp.sblk_tube[gi];
+}
+__device__ __forceinline__ WTube tube_of(const WLaunch& lp, int t) { return lp.single ? lp.tube1 : lp.tubes[t]; }
 
 __device__ __forceinline__ bool tube_skipped(const WLaunch& lp, const WTube& tube)
 {
@@ -329,8 +345,8 @@ struct ItemCtx {
 __device__ __forceinline__ ItemCtx item_ctx(const WLaunch& lp, int item, int lane, WTube& tube)
 {
     ItemCtx c;
-    c.t = __builtin_amdgcn_readfirstlane(lp.item_tube[item]);
-    tube = lp.tubes[c.t];
+    c.t = lp.single ? 0 : __builtin_amdgcn_readfirstlane(lp.item_tube[item]);
+    tube = tube_of(lp, c.t);
     // the tube is the same for the whole wavefront: keep its integers scalar
     tube.N = __builtin_amdgcn_readfirstlane(tube.N);
     tube.L = __builtin_amdgcn_readfirstlane(tube.L);
@@ -362,7 +378,8 @@ __device__ __forceinline__ void stage_segment(double* us, const double* __restri
 // into its ChainDesc for the scan and the sweeps.  DEC: the surface GF by
 // decimation (CNT.cpp:358-361) instead of the closed form.
 template <bool DEC>
-__device__ __forceinline__ void lead_sigma(const WLaunch& lp, ChainDesc* out, const ChainDesc& ch, const WTube& tube)
+__device__ __forceinline__ void lead_sigma(const WLaunch& lp, ChainDesc* out, const ChainDesc& ch, const WTube& tube,
+                                           double2* SL = nullptr, double2* SR = nullptr)
 {
     const double2 E = make_double2(ch.Er, ch.Ei);
     const double te = ch.t_even, to = tube.t_odd;
@@ -377,11 +394,17 @@ __device__ __forceinline__ void lead_sigma(const WLaunch& lp, ChainDesc* out, co
         gL = surface_gf(E, Uc0, te, to, &okL);
         gR = surface_gf(E, Uc1, te, to, &okR);
     }
+    const double t2 = to * to;
+    if (SL) {
+        *SL = make_double2(t2 * gL.x, t2 * gL.y);
+        *SR = make_double2(t2 * gR.x, t2 * gR.y);
+    }
+    if (!out) return;
     if (!okL || !okR) atomicOr(lp.status, 1);
-    out->SLr = to * to * gL.x;
-    out->SLi = to * to * gL.y;
-    out->SRr = to * to * gR.x;
-    out->SRi = to * to * gR.y;
+    out->SLr = t2 * gL.x;
+    out->SLi = t2 * gL.y;
+    out->SRr = t2 * gR.x;
+    out->SRi = t2 * gR.y;
 }
 
 // Lead self-energies by decimation, one thread per chain, before the transfer
@@ -465,7 +488,7 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
     if (item >= lp.nitems) return;
     WTube tube;
     ItemCtx ic = item_ctx(lp, item, lane, tube);
-    if (tube_skipped(lp, tube)) return;  // whole wavefront
+    if (tube.fused || tube_skipped(lp, tube)) return;  // whole wavefront
     const ChainDesc ch = lp.chains[ic.chain];
     // the first segment's wavefront also computes its chains' lead self-energies
     // in closed form (only the scan and the sweeps, later launches, read them)
@@ -553,7 +576,9 @@ __global__ __launch_bounds__(256, 4) void transfer_kernel(WLaunch lp)
 
 struct Mat2 {
     double2 a, b, c, d;  // [[a, b], [c, d]]
-    int e;               // exponent: the matrix is (a, b, c, d) 2^e
+    double e;            // exponent: the matrix is (a, b, c, d) 2^e (kept as
+                         // loaded: converting it at load time would make the
+                         // loading wave wait for each load in turn)
 };
 
 __device__ __forceinline__ Mat2 load_T(const WLaunch& lp, long long item, int lane)
@@ -564,7 +589,7 @@ __device__ __forceinline__ Mat2 load_T(const WLaunch& lp, long long item, int la
     m.b = make_double2(T[128], T[192]);
     m.c = make_double2(T[256], T[320]);
     m.d = make_double2(T[384], T[448]);
-    m.e = (int)T[512];
+    m.e = T[512];
     return m;
 }
 
@@ -575,7 +600,7 @@ __device__ __forceinline__ void mat2_apply(const Mat2& m, double2& x, double2& y
     const double2 ny = cadd(cmul(m.c, x), cmul(m.d, y));
     x = nx;
     y = ny;
-    ex += m.e;
+    ex += (int)m.e;
     renorm(x, y, ex);
 }
 
@@ -586,7 +611,7 @@ __device__ __forceinline__ void mat2_apply_raw(const Mat2& m, double2& x, double
     const double2 ny = cadd(cmul(m.c, x), cmul(m.d, y));
     x = nx;
     y = ny;
-    ex += m.e;
+    ex += (int)m.e;
 }
 
 // (x, y) <- M^{-1} (x, y) without renormalisation
@@ -596,7 +621,7 @@ __device__ __forceinline__ void mat2_apply_inv_raw(const Mat2& m, double idet, d
     const double2 ny = cscale(idet, csub(cmul(m.a, y), cmul(m.c, x)));
     x = nx;
     y = ny;
-    ex += m.e;
+    ex += (int)m.e;
 }
 
 // (x, y) <- M^{-1} (x, y) = adj(M) (x, y) / det, idet = 1/det of the true matrix
@@ -606,7 +631,7 @@ __device__ __forceinline__ void mat2_apply_inv(const Mat2& m, double idet, doubl
     const double2 ny = cscale(idet, csub(cmul(m.a, y), cmul(m.c, x)));
     x = nx;
     y = ny;
-    ex += m.e;
+    ex += (int)m.e;
     renorm(x, y, ex);
 }
 
@@ -629,10 +654,10 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
     const int gi = blockIdx.x >> 1;
     const int dir = blockIdx.x & 1;  // 0: u walk, 1: v walk
     if (gi >= ngroups) return;       // uniform over the block
-    const int t = lp.sblk_tube[gi];
-    const int g = lp.sblk_chain[gi];
-    const WTube tube = lp.tubes[t];
-    if (tube_skipped(lp, tube)) return;  // uniform over the block
+    int g;
+    const int t = group_tube(lp, gi, g);
+    const WTube tube = tube_of(lp, t);
+    if (tube.fused || tube_skipped(lp, tube)) return;  // uniform over the block
     const int P = tube.P, L = tube.L, N = tube.N;
     const int lc = g * 64 + lane;
     const bool valid = lc < tube.nchain;
@@ -727,6 +752,201 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
             lp.chain_out[(long long)ch.out_slot * 2 + 1] = GamL * GamR * iD2 * gN * gN;
         }
     }
+}
+
+// ------------------------------------------------------------ fused transfer + scan
+// Chain groups of tubes with FUSED_MIN_SEGS..FUSED_SEGS segments take one
+// launch for both steps, and their segment matrices never leave the CU (the
+// separate launches write them to memory and read them back twice, once per
+// walk).  One workgroup of FUSED_WAVES wavefronts per (64-chain group, half of
+// 32 chains).  Wave w computes the transfer matrices of segments w (lanes
+// 0-31) and w + FUSED_WAVES (lanes 32-63) of the half's chains into LDS, in
+// the arithmetic of transfer_kernel; then the first 32 lanes of wave 0 walk
+// them forward (u) and those of wave 1 backward (v), in the arithmetic of
+// scan_kernel, writing the same segment states and D.  The states are
+// renormalised every SCAN_RENORM steps of the walk (by exact powers of two,
+// so the stored (mantissa, exponent) pairs describe the same numbers for any
+// schedule).
+#define FUSED_WAVES 16
+#define FUSED_SEGS (2 * FUSED_WAVES)
+#define FUSED_MIN_SEGS 17
+
+
+// One walk over the segment matrices in LDS (V: the v walk), by the 32 lanes
+// of one wavefront, in chunks of FUSED_CHUNK steps whose matrices are read
+// from LDS together; the arithmetic is scan_kernel's.
+#define FUSED_CHUNK 4
+template <bool V>
+__device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, const double (*sig)[32],
+                                           const WTube& tube, const ChainDesc& ch, int g, int hf, int cl,
+                                           bool valid, int chain)
+{
+    const int P = tube.P, L = tube.L, N = tube.N;
+    const double te = ch.t_even, to = tube.t_odd;
+    const double tau = ((N - 1) & 1) ? to : te;  // t_{N-1}
+    const double2 SL = make_double2(sig[0][cl], sig[1][cl]);
+    const double2 SR = make_double2(sig[2][cl], sig[3][cl]);
+    double2 x, y;
+    if (!V) {
+        x = make_double2(1.0, 0.0);  // u_0
+        y = cscale(1.0 / to, SL);    // u_{-1}; t_{-1} is an odd link
+    } else {
+        x = cscale(1.0 / tau, SR);   // v_N
+        y = make_double2(1.0, 0.0);  // v_{N-1}
+    }
+    int ex = 0;
+    const double idet_odd = te / to;  // 1/det T_s when hi - 1 is even
+    const long long item0 = tube.item0 + (long long)g * P;
+    double* Sb = lp.St + item0 * 10 * 64 + (V ? 5 : 0) * 64 + hf * 32 + cl;
+    for (int i0 = 0; i0 < P; i0 += FUSED_CHUNK) {
+        Mat2 m[FUSED_CHUNK];
+#pragma unroll
+        for (int q = 0; q < FUSED_CHUNK; ++q) {
+            const int i = min(i0 + q, P - 1);
+            const double* T = Tl + (V ? P - 1 - i : i) * 9 * 32 + cl;
+            m[q].a = make_double2(T[0 * 32], T[1 * 32]);
+            m[q].b = make_double2(T[2 * 32], T[3 * 32]);
+            m[q].c = make_double2(T[4 * 32], T[5 * 32]);
+            m[q].d = make_double2(T[6 * 32], T[7 * 32]);
+            m[q].e = T[8 * 32];
+        }
+#pragma unroll
+        for (int q = 0; q < FUSED_CHUNK; ++q) {
+            if (i0 + q < P) {  // wave-uniform
+                const int sg = V ? P - 1 - (i0 + q) : i0 + q;
+                double* S = Sb + (long long)sg * 10 * 64;
+                if (!V) {
+                    S[0 * 64] = x.x;  // (u_lo, u_{lo-1})
+                    S[1 * 64] = x.y;
+                    S[2 * 64] = y.x;
+                    S[3 * 64] = y.y;
+                    S[4 * 64] = (double)ex;
+                    mat2_apply_raw(m[q], x, y, ex);
+                } else {
+                    S[0 * 64] = y.x;  // (v_{hi-1}, v_hi)
+                    S[1 * 64] = y.y;
+                    S[2 * 64] = x.x;
+                    S[3 * 64] = x.y;
+                    S[4 * 64] = (double)ex;
+                    const int hi = min(N, sg * L + L);
+                    mat2_apply_inv_raw(m[q], ((hi - 1) & 1) ? 1.0 : idet_odd, x, y, ex);
+                }
+                if (q % SCAN_RENORM == SCAN_RENORM - 1) renorm(x, y, ex);
+            }
+        }
+    }
+    if (!V && valid) {
+        // the u walk ended at the end of the chain: (u_N, u_{N-1})
+        renorm(x, y, ex);
+        const double2 Dm = csub(cscale(tau, x), cmul(SR, y));  // D = Dm 2^ex
+        double* Dp = lp.Dd + (long long)chain * 3;
+        Dp[0] = Dm.x;
+        Dp[1] = Dm.y;
+        Dp[2] = (double)ex;
+        // transmission Gamma_L Gamma_R |G_{0,N-1}|^2, G_{0,N-1} = 1/D
+        if (lp.kind == KIND_DOS && lp.chain_out && ch.out_slot >= 0) {
+            const double iD2 = 1.0 / cabs2(Dm);
+            const double gN = __builtin_amdgcn_ldexp(1.0, -ex);
+            const double GamL = -2.0 * SL.y, GamR = -2.0 * SR.y;
+            lp.chain_out[(long long)ch.out_slot * 2 + 1] = GamL * GamR * iD2 * gN * gN;
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * FUSED_WAVES, 1) void transfer_scan_kernel(WLaunch lp, int ngroups)
+{
+    __shared__ double Tl[FUSED_SEGS * 9 * 32];                // [segment][component][chain]
+    __shared__ double us_all[FUSED_WAVES][2][NEGF_SEG_MAX];   // potentials of each wave's two segments
+    __shared__ double sig[4][32];                             // SL, SR of the half's chains
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int gi = blockIdx.x >> 1;
+    const int hf = blockIdx.x & 1;
+    if (gi >= ngroups) return;  // uniform over the block
+    int g;
+    const int t = group_tube(lp, gi, g);
+    WTube tube = tube_of(lp, t);
+    tube.N = __builtin_amdgcn_readfirstlane(tube.N);
+    tube.L = __builtin_amdgcn_readfirstlane(tube.L);
+    tube.P = __builtin_amdgcn_readfirstlane(tube.P);
+    tube.nchain = __builtin_amdgcn_readfirstlane(tube.nchain);
+    // (a half without chains still runs, on copies of the group's last chain:
+    // the sweeps read the states of all 64 lanes and mask the contributions)
+    if (!tube.fused || tube_skipped(lp, tube)) return;       // uniform over the block
+    const int P = tube.P, L = tube.L, N = tube.N;
+    const int h = lane >> 5, cl = lane & 31;
+    const int lc = g * 64 + hf * 32 + cl;
+    const bool valid = lc < tube.nchain;
+    const int chain = tube.chain0 + (valid ? lc : tube.nchain - 1);
+    const ChainDesc ch = lp.chains[chain];
+    const double2 E = make_double2(ch.Er, ch.Ei);
+    const double te = ch.t_even, to = tube.t_odd;
+
+    // potentials of the wave's two segments (zero past the end of the tube)
+    for (int i = lane; i < 2 * L; i += 64) {
+        const int hh = i >= L ? 1 : 0;
+        const int j = i - hh * L;
+        const int site = (w + FUSED_WAVES * hh) * L + j;
+        us_all[w][hh][j] = site < N ? tube.U[site] : 0.0;
+    }
+    // lead self-energies: the lanes of segment 0 (closed form, also stored for
+    // the sweeps), or as the decimation launch left them in the chain table
+    if (w == 0 && h == 0) {
+        double2 SL = make_double2(ch.SLr, ch.SLi), SR = make_double2(ch.SRr, ch.SRi);
+        if (!lp.dec_use) lead_sigma<false>(lp, valid ? lp.chains + chain : nullptr, ch, tube, &SL, &SR);
+        sig[0][cl] = SL.x;
+        sig[1][cl] = SL.y;
+        sig[2][cl] = SR.x;
+        sig[3][cl] = SR.y;
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    const int s = w + FUSED_WAVES * h;  // this lane's segment
+    const int lo = min(N, s * L);
+    const int n = s < P ? min(N, lo + L) - lo : 0;
+    const double* us = us_all[w][h];
+    double2 m00 = make_double2(1.0, 0.0), m01 = make_double2(0.0, 0.0);
+    double2 m10 = make_double2(0.0, 0.0), m11 = make_double2(1.0, 0.0);
+    int eT = 0;
+    if (__all(scaled_ok(E, te, to))) {
+        const SCoefs ks = make_scoefs(E, te, to);
+        if (n > 0) {
+            const int hi = lo + n;
+            m11 = make_double2(((lo - 1) & 1) ? ks.ai[1] : ks.ai[0], 0.0);
+            transfer_loop<NEGF_SUB_K>(ks, us, n, m00, m01, m10, m11, eT);
+            double mC;
+            int eC;
+            scale_prod(ks, lo & 1, n - 1, mC, eC);
+            const double r0 = mC * (((hi - 1) & 1) ? ks.ai[1] : ks.ai[0]);
+            m00 = cscale(r0, m00);
+            m01 = cscale(r0, m01);
+            m10 = cscale(mC, m10);
+            m11 = cscale(mC, m11);
+            eT += eC;
+            renorm_mat(m00, m01, m10, m11, eT);
+        }
+    } else {
+        const Coefs k = make_coefs(E, te, to);
+        if (n > 0) transfer_loop<NEGF_SUB_K>(k, us, n, m00, m01, m10, m11, eT);
+    }
+    if (s < P) {
+        double* T = Tl + s * 9 * 32 + cl;
+        T[0 * 32] = m00.x;
+        T[1 * 32] = m00.y;
+        T[2 * 32] = m01.x;
+        T[3 * 32] = m01.y;
+        T[4 * 32] = m10.x;
+        T[5 * 32] = m10.y;
+        T[6 * 32] = m11.x;
+        T[7 * 32] = m11.y;
+        T[8 * 32] = (double)eT;
+    }
+    __syncthreads();
+    if (w >= 2 || lane >= 32) return;
+    if (w == 0)
+        fused_walk<false>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain);
+    else
+        fused_walk<true>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain);
 }
 
 // ------------------------------------------------------------ sweep

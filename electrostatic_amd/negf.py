@@ -213,6 +213,16 @@ class Batch:
         _lib.check(self.lib.negf_batch_autotune(self.h, C.c_void_p(dQ_ptr), C.byref(L), C.byref(ms)))
         return L.value, ms.value
 
+    def autotune_cached(self, dQ_ptr, cache_path):
+        """Segment length from the tuning cache at cache_path (tuned and
+        appended on a miss); returns (L, ms, from_cache)."""
+        L = C.c_int()
+        ms = C.c_double()
+        hit = C.c_int()
+        _lib.check(self.lib.negf_batch_autotune_cached(self.h, C.c_void_p(dQ_ptr), str(cache_path).encode(),
+                                                       C.byref(L), C.byref(ms), C.byref(hit)))
+        return L.value, ms.value, bool(hit.value)
+
     def enable_timing(self, on=True):
         _lib.check(self.lib.negf_batch_enable_timing(self.h, int(on)))
 

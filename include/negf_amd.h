@@ -275,6 +275,13 @@ int negf_batch_set_segment_length(negf_batch *b, int segment_length);
 /* Time the solve for segment lengths 32..128 and keep the fastest. */
 int negf_batch_autotune(negf_batch *b, double *d_Q, int *segment_length,
                         double *ms);
+/* negf_batch_autotune with a persistent cache (no reference counterpart):
+ * cache_path is a text file of "key segment_length ms" lines, the key naming
+ * the device (architecture, compute units) and a hash of every tube's sites
+ * and chain counts.  A hit applies the stored length without timing
+ * (*from_cache = 1); a miss tunes and appends the result. */
+int negf_batch_autotune_cached(negf_batch *b, double *d_Q, const char *cache_path,
+                               int *segment_length, double *ms, int *from_cache);
 /* hipEvent time of the last solve's launch sequence. */
 int negf_batch_enable_timing(negf_batch *b, int on);
 int negf_batch_last_time(negf_batch *b, double *ms);

@@ -263,3 +263,36 @@ def test_batched_scf_rejects_bad_host_inputs(dev):
         scf.run(0.2)                      # one gate value for three tubes
     scf.close()
     batch.close()
+
+
+def test_batch_autotune_cache(dev, tmp_path):
+    # a miss tunes and appends one line; a second batch of the same workload
+    # takes the stored length without timing; a different workload misses
+    cache = tmp_path / "tune.txt"
+    tubes, _ = make_tubes([30, 7, 80], [0.15, 0.0, 0.3], modes=1, seed=5)
+    batch = negf.Batch(tubes)
+    n = int(batch.offsets[-1])
+    d = torch.zeros(n, dtype=torch.float64, device=dev)
+    L1, ms1, hit1 = batch.autotune_cached(d.data_ptr(), cache)
+    assert not hit1 and L1 in (32, 48, 64, 96, 128) and ms1 > 0
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].split()[-2] == str(L1)
+    batch.solve_device(1, d.data_ptr())
+    torch.cuda.synchronize()
+    ref = d.cpu().numpy()
+    batch.close()
+    tubes2, _ = make_tubes([30, 7, 80], [0.15, 0.0, 0.3], modes=1, seed=5)
+    batch2 = negf.Batch(tubes2)
+    L2, ms2, hit2 = batch2.autotune_cached(d.data_ptr(), cache)
+    assert hit2 and L2 == L1 and ms2 == pytest.approx(ms1, rel=1e-5)
+    d2 = torch.zeros(n, dtype=torch.float64, device=dev)
+    batch2.solve_device(2, d2.data_ptr())
+    torch.cuda.synchronize()
+    assert rel(d2.cpu().numpy(), ref) < 1e-14
+    batch2.close()
+    tubes3, _ = make_tubes([30, 8, 80], [0.15, 0.0, 0.3], modes=1, seed=5)
+    batch3 = negf.Batch(tubes3)
+    d3 = torch.zeros(int(batch3.offsets[-1]), dtype=torch.float64, device=dev)
+    _, _, hit3 = batch3.autotune_cached(d3.data_ptr(), cache)
+    assert not hit3 and len(cache.read_text().splitlines()) == 2
+    batch3.close()

@@ -421,3 +421,67 @@ def test_band_edge_noneq_matches_oracle(eta, flat):
     err = rel_err(to_np(acc), ref)
     print(f"eta={eta:g} flat={flat}: noneq vs oracle {err:.2e}")
     assert err < (3e-10 if flat else 1e-10)
+
+
+def with_fused(flag, fn):
+    """fn() with the fused transfer+scan launch switched on or off (the
+    library reads NEGF_RGF_FUSED whenever it builds a launch plan)."""
+    import os
+    old = os.environ.get("NEGF_RGF_FUSED")
+    os.environ["NEGF_RGF_FUSED"] = flag
+    try:
+        return fn()
+    finally:
+        if old is None:
+            os.environ.pop("NEGF_RGF_FUSED", None)
+        else:
+            os.environ["NEGF_RGF_FUSED"] = old
+
+
+@pytest.mark.parametrize("N,L", [(2100, 0), (4096, 0), (1000, 48), (777, 32)])
+def test_fused_transfer_scan_matches_separate_launches(N, L):
+    """Tubes of 17..32 segments take one launch for transfer matrices and
+    scan (matrices kept in LDS); it must give what the separate launches give
+    for every launch kind: charge with and without diag G^R, lead self-energies
+    in closed form and by decimation, non-equilibrium charge with and without
+    integrand samples, DOS and transmission.  Tolerance 1e-13 relative: the v
+    walk applies the adjugate as one matrix-vector product, which may round
+    differently from the separate scan's subtraction form."""
+    lib = _lib.load()
+    U = smooth_potential(N, seed=N + 1, lo=-0.3, hi=0.3)
+    E = energies(40, 40, seed=N)
+    rng = np.random.default_rng(N)
+    coef = rng.normal(size=E.size) + 1j * rng.normal(size=E.size)
+    Uc = (U[0], U[-1])
+    Er = np.linspace(-2.5, 1.5, 48) + 1e-5j
+    cL = rng.normal(size=Er.size)
+    cR = rng.normal(size=Er.size)
+
+    def run():
+        out = {}
+        out["eqG"] = run_diag(U, BETA17, GAMMA, Uc, E, coef, segment_length=L)
+        out["eq"] = run_diag(U, BETA17, GAMMA, Uc, E, coef, want_G=False, segment_length=L)[0]
+        out["dec"] = run_diag(U, BETA17, GAMMA, Uc, E, coef, want_G=False, segment_length=L,
+                              use_decimation=1, layers=10, max_iter=30, rel_error=0)[0]
+        p = chain_params(N, BETA17, GAMMA, Uc, segment_length=L)
+        dU, dE, dL, dR = dev_array(U), dev_array(Er), dev_array(cL), dev_array(cR)
+        acc, acc2, integ = dev_zeros(N), dev_zeros(N), dev_zeros(3 * Er.size)
+        _lib.check(lib.negf_rgf_noneq_accumulate(p, ptr(dU), Er.size, ptr(dE), ptr(dL), ptr(dR),
+                                                 None, None, ptr(acc), None, None))
+        _lib.check(lib.negf_rgf_noneq_accumulate(p, ptr(dU), Er.size, ptr(dE), ptr(dL), ptr(dR),
+                                                 None, None, ptr(acc2), ptr(integ), None))
+        out["neq"], out["neq2"], out["integ"] = to_np(acc), to_np(acc2), to_np(integ)
+        dd, dt = dev_zeros(Er.size), dev_zeros(Er.size)
+        _lib.check(lib.negf_rgf_dos(p, ptr(dU), Er.size, ptr(dE), ptr(dd), ptr(dt), None))
+        out["dos"], out["tr"] = to_np(dd), to_np(dt)
+        return out
+
+    fused = with_fused("1", run)
+    sep = with_fused("0", run)
+    assert rel_err(fused["eqG"][0], sep["eqG"][0]) < 1e-13
+    assert np.abs(fused["eqG"][1] - sep["eqG"][1]).max() < 1e-13 * np.abs(sep["eqG"][1]).max()
+    for key in ("eq", "dec", "neq", "neq2", "integ", "dos", "tr"):
+        assert rel_err(fused[key], sep[key]) < 1e-13, key
+    # and against the oracle
+    acc_ref, st = O.accumulate_diagG(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E, coef)
+    assert st == 0 and rel_err(fused["eq"], acc_ref) < 1e-10
