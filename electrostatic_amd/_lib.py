@@ -153,6 +153,16 @@ SIGNATURES = {
     "negf_block_self_energies": (C.c_int, [vp, C.c_int, vp, C.c_double, C.c_int, dp, ip]),
     "negf_block_probe": (C.c_int, [C.c_int, vp, vp, vp]),
     "negf_block_chain_set_scratch_limit": (C.c_int, [vp, C.c_size_t]),
+    "negf_block_chain_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "negf_block_material_create": (C.c_int, [vp, C.POINTER(negf_tube_params), C.c_double, C.c_double, C.c_int,
+                                             C.POINTER(vp)]),
+    "negf_block_material_destroy": (C.c_int, [vp]),
+    "negf_block_material_set_potential": (C.c_int, [vp, dp, dp]),
+    "negf_block_material_set_contacts": (C.c_int, [vp, dp, dp]),
+    "negf_block_material_rho0": (C.c_int, [vp, dp]),
+    "negf_block_solve": (C.c_int, [vp, dp, C.POINTER(C.c_int)]),
+    "negf_block_scf_bias_point": (C.c_int, [vp, dp, dp, C.c_double, C.POINTER(C.c_int), C.POINTER(negf_scf_params),
+                                            C.c_int, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     "negf_block_last_time": (C.c_int, [vp, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     "negf_batch_scf_get_kernel": (C.c_int, [vp, dp]),
     "negf_batch_scf_set_preconditioner": (C.c_int, [vp, dp, dp]),

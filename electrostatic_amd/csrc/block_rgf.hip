@@ -816,6 +816,7 @@ __global__ void block_reduce_kernel(int nE, int len, const double* __restrict__ 
 
 struct BlockChain {
     int nb = 0;
+    int m = BM;  // orbitals per block of the caller; the device blocks are BM wide (padded)
     std::vector<double> H0, T;
     double Uc[2] = {0.0, 0.0};
     DevBuf d_H0, d_T, d_v, d_E, d_coef, d_Sigma, d_iters, d_g, d_part, d_out, d_diag;
@@ -844,24 +845,39 @@ struct negf_block_chain {
 
 extern "C" {
 
+// Blocks of m < BM orbitals run as BM-wide blocks: the extra orbitals have
+// the on-site energy kPadLevel and no couplings, so they are decoupled from
+// the caller's orbitals (G is block diagonal in the two sets), their Green's
+// function 1 / (E - kPadLevel - Sigma = 0) stays finite on every path, and
+// their outputs are dropped.
+static constexpr double kPadLevel = 1.0e4;  // eV
+
 int negf_block_chain_create(int m, int nblocks, const double* H0, const double* T, negf_block_chain** out)
 {
     clear_error();
-    if (m != BM || nblocks < 1 || !H0 || !T || !out) {
-        set_error("block chain needs m = %d, nblocks >= 1 and H0, T", BM);
+    if (m < 1 || m > BM || nblocks < 1 || !H0 || !T || !out) {
+        set_error("block chain needs 1 <= m <= %d, nblocks >= 1 and H0, T", BM);
         return NEGF_EARG;
     }
-    for (int i = 0; i < BM; ++i)
-        for (int j = 0; j < BM; ++j)
-            if (H0[i * BM + j] != H0[j * BM + i]) {
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j)
+            if (H0[i * m + j] != H0[j * m + i]) {
                 set_error("H0 must be symmetric");
                 return NEGF_EARG;
             }
     negf_block_chain* h = new negf_block_chain;
     BlockChain& c = h->c;
     c.nb = nblocks;
-    c.H0.assign(H0, H0 + BM * BM);
-    c.T.assign(T, T + BM * BM);
+    c.m = m;
+    c.H0.assign(BM * BM, 0.0);
+    c.T.assign(BM * BM, 0.0);
+    for (int i = 0; i < BM; ++i)
+        for (int j = 0; j < BM; ++j) {
+            c.H0[i * BM + j] = (i < m && j < m) ? H0[i * m + j] : (i == j ? kPadLevel : 0.0);
+            c.T[i * BM + j] = (i < m && j < m) ? T[i * m + j] : 0.0;
+        }
+    H0 = c.H0.data();
+    T = c.T.data();
     int rc = 0;
     auto fail = [&](int r) {
         delete h;
@@ -889,6 +905,14 @@ int negf_block_chain_create(int m, int nblocks, const double* H0, const double* 
     return NEGF_OK;
 }
 
+int negf_block_chain_dims(const negf_block_chain* h, int* m, int* nblocks)
+{
+    if (!h || !m || !nblocks) return NEGF_EARG;
+    *m = h->c.m;
+    *nblocks = h->c.nb;
+    return NEGF_OK;
+}
+
 int negf_block_chain_destroy(negf_block_chain* h)
 {
     if (!h) return NEGF_EARG;
@@ -903,7 +927,14 @@ int negf_block_chain_set_potential(negf_block_chain* h, const double* v, const d
         set_error("null argument");
         return NEGF_EARG;
     }
-    NEGF_HIP_TRY(hipMemcpy(h->c.d_v.p, v, sizeof(double) * (size_t)h->c.nb * BM, hipMemcpyHostToDevice));
+    const int m = h->c.m, nb = h->c.nb;
+    if (m == BM) {
+        NEGF_HIP_TRY(hipMemcpy(h->c.d_v.p, v, sizeof(double) * (size_t)nb * BM, hipMemcpyHostToDevice));
+    } else {
+        std::vector<double> pad((size_t)nb * BM, 0.0);
+        for (int b = 0; b < nb; ++b) std::copy(v + (size_t)b * m, v + (size_t)(b + 1) * m, pad.begin() + (size_t)b * BM);
+        NEGF_HIP_TRY(hipMemcpy(h->c.d_v.p, pad.data(), sizeof(double) * pad.size(), hipMemcpyHostToDevice));
+    }
     h->c.Uc[0] = Uc[0];
     h->c.Uc[1] = Uc[1];
     return NEGF_OK;
@@ -963,10 +994,21 @@ int negf_block_rgf(negf_block_chain* h, int nE, const negf_complex* E, const neg
     NEGF_HIP_TRY(hipGetLastError());
     std::vector<int> it(2 * nE);
     NEGF_HIP_TRY(hipMemcpyAsync(it.data(), c.d_iters.p, sizeof(int) * 2 * nE, hipMemcpyDeviceToHost, c.st));
-    if (acc) NEGF_HIP_TRY(hipMemcpyAsync(acc, c.d_out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c.st));
-    if (diagG)
-        NEGF_HIP_TRY(hipMemcpyAsync(diagG, c.d_diag.p, sizeof(double2) * (size_t)nE * len, hipMemcpyDeviceToHost,
-                                    c.st));
+    const int m = c.m;
+    if (m == BM) {
+        if (acc) NEGF_HIP_TRY(hipMemcpyAsync(acc, c.d_out.p, sizeof(double) * len, hipMemcpyDeviceToHost, c.st));
+        if (diagG)
+            NEGF_HIP_TRY(hipMemcpyAsync(diagG, c.d_diag.p, sizeof(double2) * (size_t)nE * len,
+                                        hipMemcpyDeviceToHost, c.st));
+    } else {
+        // the caller's m orbitals of every BM-wide block (rows of BM, first m)
+        if (acc)
+            NEGF_HIP_TRY(hipMemcpy2DAsync(acc, sizeof(double) * m, c.d_out.p, sizeof(double) * BM,
+                                          sizeof(double) * m, nb, hipMemcpyDeviceToHost, c.st));
+        if (diagG)
+            NEGF_HIP_TRY(hipMemcpy2DAsync(diagG, sizeof(double2) * m, c.d_diag.p, sizeof(double2) * BM,
+                                          sizeof(double2) * m, (size_t)nE * nb, hipMemcpyDeviceToHost, c.st));
+    }
     NEGF_HIP_TRY(hipStreamSynchronize(c.st));
     NEGF_HIP_TRY(hipEventElapsedTime(&c.ms_sr, c.ev[0], c.ev[1]));
     NEGF_HIP_TRY(hipEventElapsedTime(&c.ms_rgf, c.ev[1], c.ev[2]));
@@ -1001,9 +1043,21 @@ int negf_block_self_energies(negf_block_chain* h, int nE, const negf_complex* E,
     NEGF_HIP_TRY(hipGetLastError());
     std::vector<int> it(2 * nE);
     NEGF_HIP_TRY(hipMemcpyAsync(it.data(), c.d_iters.p, sizeof(int) * 2 * nE, hipMemcpyDeviceToHost, c.st));
-    NEGF_HIP_TRY(hipMemcpyAsync(sigma, c.d_Sigma.p, sizeof(double) * (size_t)nE * 4 * BGL, hipMemcpyDeviceToHost,
-                                c.st));
+    const int m = c.m;
+    if (m == BM) {
+        NEGF_HIP_TRY(hipMemcpyAsync(sigma, c.d_Sigma.p, sizeof(double) * (size_t)nE * 4 * BGL,
+                                    hipMemcpyDeviceToHost, c.st));
+    }
+    std::vector<double> full(m == BM ? 0 : (size_t)nE * 4 * BGL);
+    if (m != BM)
+        NEGF_HIP_TRY(hipMemcpyAsync(full.data(), c.d_Sigma.p, sizeof(double) * full.size(), hipMemcpyDeviceToHost,
+                                    c.st));
     NEGF_HIP_TRY(hipStreamSynchronize(c.st));
+    // the m x m leading submatrix of every [energy][lead][plane] matrix
+    for (size_t q = 0; m != BM && q < (size_t)nE * 4; ++q)
+        for (int i = 0; i < m; ++i)
+            std::copy(full.data() + q * BGL + (size_t)i * BM, full.data() + q * BGL + (size_t)i * BM + m,
+                      sigma + q * m * m + (size_t)i * m);
     if (sr_iters) std::copy(it.begin(), it.end(), sr_iters);
     for (int k = 0; k < 2 * nE; ++k)
         if (it[k] <= 0) {

@@ -24,6 +24,10 @@ struct Path {
 };
 
 void compute_limits(const negf_tube_params& p, const double mu[2], const double T[2], Limits& L);
+// Gauss-Legendre path from a to b (type 0 line, 1 arc) appended to out.
+void add_path(Path& out, std::complex<double> a, std::complex<double> b, int n, int type);
+// FermiFunction (NEGF_Common.cpp:1788-1794) at complex E - mu.
+std::complex<double> fermi(std::complex<double> E_minus_mu, double kT);
 int fill_device(double* p, long long n, double v, hipStream_t st);
 
 class Tube {

@@ -173,9 +173,9 @@ void compute_limits(const negf_tube_params& p, const double mu[2], const double 
 }
 
 // FermiFunction (NEGF_Common.cpp:1788-1794): 1 / (exp((E - mu)/kT) + 1).
-static cplx fermi(cplx E_minus_mu, double kT) { return 1.0 / (std::exp(E_minus_mu / kT) + 1.0); }
+cplx fermi(cplx E_minus_mu, double kT) { return 1.0 / (std::exp(E_minus_mu / kT) + 1.0); }
 
-static void add_path(Path& out, cplx a, cplx b, int n, int type)
+void add_path(Path& out, cplx a, cplx b, int n, int type)
 {
     if (n <= 0) return;
     const size_t k0 = out.E.size();

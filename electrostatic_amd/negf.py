@@ -548,17 +548,18 @@ def conductance(E, T, mu0, kT0):
 
 
 class BlockChain:
-    """Chain of dense 32 x 32 blocks (negf_block_*): on-site H0 + diag(v_b),
+    """Chain of dense m x m blocks, m <= 32 (negf_block_*): on-site H0 + diag(v_b),
     coupling T = H_{b,b+1}, leads continuing the chain; Sancho-Rubio lead
     surface Green's functions and the block RGF sweep on FP64 MFMA."""
-
-    M = 32
 
     def __init__(self, nblocks, H0, T):
         self.lib = _lib.load()
         self.nb = nblocks
         H0 = np.ascontiguousarray(H0, np.float64)
         T = np.ascontiguousarray(T, np.float64)
+        self.M = H0.shape[0]   # orbitals per block (at most 32)
+        if H0.shape != (self.M, self.M) or T.shape != (self.M, self.M):
+            raise ValueError("H0 and T must be square of the same size")
         h = C.c_void_p()
         _lib.check(self.lib.negf_block_chain_create(self.M, nblocks, _dp(H0), _dp(T), C.byref(h)))
         self.h = h
@@ -615,3 +616,70 @@ class BlockChain:
         b = C.c_float()
         _lib.check(self.lib.negf_block_last_time(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class BlockMaterial:
+    """Equilibrium charge and self-consistent loop of a BlockChain with the
+    tube material's contour, poles and Rho0 (negf_block_material_*).  The
+    contour settings are tube input keys (contact_mu, contact_T,
+    E_valence_min, E_pole_max, E_zPlus_imag, eq_integration_pts, ...)."""
+
+    def __init__(self, chain, spin_degen=2.0, sr_tol=1e-12, sr_max_iter=200, params=None, **keys):
+        self.lib = _lib.load()
+        self.chain = chain
+        self.S = chain.nb * chain.M
+        self.params = params if params is not None else tube_params(**keys)
+        h = C.c_void_p()
+        _lib.check(self.lib.negf_block_material_create(chain.h, C.byref(self.params), float(spin_degen),
+                                                       float(sr_tol), int(sr_max_iter), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.negf_block_material_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_potential(self, v, Uc=(0.0, 0.0)):
+        v = np.ascontiguousarray(v, np.float64).reshape(self.S)
+        uc = np.ascontiguousarray(Uc, np.float64)
+        _lib.check(self.lib.negf_block_material_set_potential(self.h, _dp(v), _dp(uc)))
+
+    def set_contacts(self, mu, T=(298.0, 298.0)):
+        m = np.ascontiguousarray(mu, np.float64)
+        t = np.ascontiguousarray(T, np.float64)
+        _lib.check(self.lib.negf_block_material_set_contacts(self.h, _dp(m), _dp(t)))
+
+    def rho0(self):
+        out = np.zeros(self.S)
+        _lib.check(self.lib.negf_block_material_rho0(self.h, _dp(out)))
+        return out.reshape(self.chain.nb, self.chain.M)
+
+    def solve(self):
+        """(Q [nb, 32] = RhoEq - Rho0, number of integration points)."""
+        Q = np.zeros(self.S)
+        n = C.c_int()
+        _lib.check(self.lib.negf_block_solve(self.h, _dp(Q), C.byref(n)))
+        return Q.reshape(self.chain.nb, self.chain.M), n.value
+
+    def scf_bias_point(self, K, V_gate, U_ext=None, contact_orbitals=None, max_iters=100, params=None, **keys):
+        """Self-consistent (U, Q, iterations, final norm) with
+        U = U_ext - V_gate - K n (K: S x S); the leads take U of
+        contact_orbitals (first block, last block) or the end blocks' mean."""
+        sp = params if params is not None else scf_params(**keys)
+        co = None if contact_orbitals is None else (C.c_int * 2)(*[int(x) for x in contact_orbitals])
+        K = np.ascontiguousarray(K, np.float64).reshape(self.S, self.S)
+        ue = None if U_ext is None else np.ascontiguousarray(U_ext, np.float64).reshape(self.S)
+        U = np.zeros(self.S)
+        Q = np.zeros(self.S)
+        it = C.c_int()
+        nrm = C.c_double()
+        _lib.check(self.lib.negf_block_scf_bias_point(self.h, _dp(K), None if ue is None else _dp(ue), float(V_gate),
+                                                      co, C.byref(sp), int(max_iters), _dp(U), _dp(Q), C.byref(it),
+                                                      C.byref(nrm)))
+        return U, Q, it.value, nrm.value

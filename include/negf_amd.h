@@ -401,10 +401,12 @@ int negf_multi_scf_get_input_charge(const negf_multi_scf *s, double *n_in);
  * leaves it stubbed: the 2-D MatrixBlock multiply/inverse/dagger are empty
  * (Matrix_Block_Util.H:123-135, 431-438, 684-692) and DecimationTechnique
  * (NEGF_Common.cpp:4095-4153) works only for 1-D blocks.  A chain of
- * nblocks blocks of m = 32 orbitals has on-site H0 + diag(v_b) (H0 real
+ * nblocks blocks of 1 <= m <= 32 orbitals has on-site H0 + diag(v_b) (H0 real
  * symmetric m x m, row-major; v_b real, per orbital) and the same real
  * coupling T = H_{b,b+1} between consecutive blocks; the leads continue the
- * chain with on-site H0 + Uc[0] (left) and H0 + Uc[1] (right).
+ * chain with on-site H0 + Uc[0] (left) and H0 + Uc[1] (right).  Blocks of
+ * m < 32 run as 32-wide blocks whose extra orbitals are decoupled (on-site
+ * 1e4 eV, no couplings) and left out of every output, at the cost of m = 32.
  * ------------------------------------------------------------------------ */
 typedef struct negf_block_chain negf_block_chain;
 int negf_block_chain_create(int m, int nblocks, const double *H0, const double *T,
@@ -441,6 +443,48 @@ int negf_block_last_time(const negf_block_chain *c, float *ms_sancho_rubio, floa
  * least one energy per sweep).  Results do not depend on it.  No reference
  * counterpart. */
 int negf_block_chain_set_scratch_limit(negf_block_chain *c, size_t bytes);
+/* m (orbitals per block) and nblocks of a chain. */
+int negf_block_chain_dims(const negf_block_chain *c, int *m, int *nblocks);
+
+/* Block material: the tube material's equilibrium contour, enclosed poles,
+ * Rho0 and charge combine over a block chain (Compute_RhoEq
+ * NEGF_Common.cpp:3667-3754 on the paths of :1796-1800, Compute_GR_atPoles
+ * :3833-3952, Compute_Rho0 :3957-4092, Q = RhoEq - Rho0 as in Solve_NEGF).
+ * The contour settings come from p (contact_mu, contact_T, E_valence_min,
+ * E_pole_max, E_zPlus_imag, Fermi_tail_factors, eq_integration_pts; the
+ * tube-geometry fields are ignored); spin_degen multiplies every orbital.
+ * Equilibrium contacts only: NEGF_ENOTSUP when |mu_L - mu_R| opens the
+ * non-equilibrium window (the block sweep forms only diagonal blocks of G).
+ * The chain c must outlive the material. */
+typedef struct negf_block_material negf_block_material;
+int negf_block_material_create(negf_block_chain *c, const negf_tube_params *p,
+                               double spin_degen, double sr_tol, int sr_max_iter,
+                               negf_block_material **out);
+int negf_block_material_destroy(negf_block_material *m);
+/* v: [nblocks][m] orbital potentials; Uc: lead potentials. */
+int negf_block_material_set_potential(negf_block_material *m, const double *v,
+                                      const double Uc[2]);
+int negf_block_material_set_contacts(negf_block_material *m, const double mu[2],
+                                     const double T[2]);
+/* Rho0 per orbital (computed at creation, U = 0, mu = 0). */
+int negf_block_material_rho0(const negf_block_material *m, double *rho0);
+/* Q[b*m + o] = RhoEq - Rho0 at the current potential (Solve_NEGF,
+ * Nanostructure.H:115-120); n_intg_pts: the contour's points
+ * (get_Total_Integration_Pts, NEGF_Common.cpp:1826-1841; poles not counted). */
+int negf_block_solve(negf_block_material *m, double *Q, int *n_intg_pts);
+/* One bias point of the self-consistent loop over the orbitals with the
+ * linear stand-in U_i = U_ext_i - V_gate - sum_j K_ij n_in_j (K: S x S
+ * row-major, S = nblocks*m; U_ext may be NULL), lead potentials U of
+ * orbital contact_orbital[0] of the first block and contact_orbital[1] of
+ * the last (NULL: the end blocks' mean U), Broyden modified-second on n
+ * (Broyden_* fields of sp; reset_with_previous_charge_distribution starts
+ * from the last bias point's charge).  U_out, Q_out: S values;
+ * NEGF_ENONCONV after max_iters.  No reference counterpart beyond the
+ * loop of c_TransportSolver::Solve (Transport.cpp:359-539). */
+int negf_block_scf_bias_point(negf_block_material *m, const double *K, const double *U_ext,
+                              double V_gate, const int *contact_orbital,
+                              const negf_scf_params *sp, int max_iters,
+                              double *U_out, double *Q_out, int *iters, double *final_norm);
 
 /* ------------------------------------------------------------------------
  * Charge surrogate (no reference counterpart; a new component next to the
