@@ -143,7 +143,23 @@ def scf_sweep(with_cpu=True):
     return out
 
 
-def batch_leg(dev, steps=10):
+def copy_calibration_leg(dev, mib=256, reps=3):
+    """A plain device-to-device copy of mib MiB, reps times: the known-size
+    traffic that scales the FETCH_SIZE and WRITE_SIZE counters."""
+    import torch
+    n = (mib << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"bytes_read": n * 8, "bytes_written": n * 8, "reps": reps, "GB_per_s": 2 * n * 8 / dt / 1e9}
+
+
+def batch_leg(dev, steps=10, segment_length=0):
     """64 (17,0) tubes, lengths log-uniform in [500, 20000] sites (seed 3),
     2 modes each, 256 energies per tube (contour 32/32/64 + 128 real-axis
     points for the tube's own bias V ~ U(0, 0.5) V, contacts at -1 and -1-V eV),
@@ -165,7 +181,11 @@ def batch_leg(dev, steps=10):
     batch = negf.Batch(tubes)
     n = int(batch.offsets[-1])
     dQ = torch.zeros(n, dtype=torch.float64, device=dev)
-    L, tune_ms = batch.autotune(dQ.data_ptr())
+    if segment_length > 0:
+        batch.set_segment_length(segment_length)
+        L = segment_length
+    else:
+        L, tune_ms = batch.autotune(dQ.data_ptr())
     es = batch.energy_sites()
     out = {"workload": "64 uneven (17,0) tubes, N log-uniform [500, 20000] (seed 3), 2 modes, "
                        "256 energies per tube (32/32/64 contour + 128 real axis), per-tube bias U(0, 0.5) V",
@@ -395,12 +415,24 @@ def main():
                     help="also time the surrogate's forward and tangent passes alone")
     ap.add_argument("--write-diag-g", action="store_true",
                     help="also materialise diag G^R ([energy][site], 512 MiB)")
+    ap.add_argument("--only", choices=["batch", "block", "surrogate-kernels", "copy-calibration"],
+                    help="run one secondary leg alone and print its figures (for kernel traces and "
+                         "counters; --segment-length then fixes the batch leg's segment length)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the energy-sharded multi-GPU form is a design note, not a run)",
               file=sys.stderr)
         sys.exit(2)
+    if args.only:
+        import torch
+        dev = torch.device("cuda:0")
+        legs = {"batch": lambda: batch_leg(dev, segment_length=args.segment_length),
+                "block": lambda: block_leg(with_cpu=False),
+                "surrogate-kernels": lambda: surrogate_kernels_leg(dev),
+                "copy-calibration": lambda: copy_calibration_leg(dev)}
+        print(json.dumps({"only": args.only, "result": legs[args.only]()}), flush=True)
+        return
 
     import torch
     from electrostatic_amd import _lib
