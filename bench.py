@@ -30,7 +30,7 @@ METRIC = "RGF energy·site/s (% FP64 peak); SCF bias points/s surrogate vs plain
 FLOP_PER_ENERGY_SITE = 31          # algorithmic FP64 flops (add/mul/div = 1, FMA = 2)
 FP64_PEAK_TFLOPS = 78.6            # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0              # MI355X HBM3E
-HOT_PATH_KERNELS = "transfer_scan_kernel+sweep_eq_full_kernel (RGF hot path; transfer_kernel+scan_kernel for tubes outside 17..32 segments)"
+HOT_PATH_KERNELS = "rgf_eq_fused_kernel (RGF hot path: transfer matrices, walks and equilibrium sweep in one launch)"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rgf_seg_batched_traffic.json")
 
 

@@ -223,7 +223,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     max_N_ = 0;
     max_P_ = 0;
     max_groups_ = 0;
-    any_fused_ = any_unfused_ = false;
+    any_fused_ = any_fused2_ = any_unfused_ = false;
     for (int t = 0; t < ntubes_; ++t) {
         const RgfTube& tb = tubes[t];
         if (tb.N < 2 || !tb.dU || tb.nchain <= 0 || tb.t_odd <= 0 || tb.chain0 < 0 ||
@@ -239,10 +239,14 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         }
         WTube& w = wt[t];
         w = make_wtube(tb, P, L, items, poff);
-        (w.fused ? any_fused_ : any_unfused_) = true;
+        // equilibrium charge on full 128-site segments: the whole launch fused
+        if (w.fused && kind == KIND_EQ && !write_G && L == NEGF_SEG_MAX && tb.N % L == 0) w.fused = 2;
+        (w.fused == 2 ? any_fused2_ : w.fused ? any_fused_ : any_unfused_) = true;
         const int G = (tb.nchain + 63) / 64;
+        const int rows = w.fused == 2 ? 2 * G : G;  // partial rows: one per 32 chains when fused
         for (int i = 0; i < G * P; ++i) {
             item_tube.push_back(t);
+            if (w.fused == 2) continue;  // no separate sweep
             // full segments of the equilibrium charge go to the unrolled sweep
             const int sgi = i % P;
             const bool full = sgi * L + L <= tb.N;
@@ -260,9 +264,9 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
             sblk_chain.push_back(g);
         }
         rsrc.push_back(poff);
-        rnrows.push_back(G);
+        rnrows.push_back(rows);
         rN.push_back(tb.N);
-        poff += (long long)G * tb.N;
+        poff += (long long)rows * tb.N;
         rdst.push_back(tb.out_off);
         max_N_ = std::max(max_N_, tb.N);
         max_P_ = std::max(max_P_, P);
@@ -328,6 +332,7 @@ struct RgfLayout {
     const long long* red_dst_off = nullptr;
     DecParams dec;
     bool any_fused = false, any_unfused = true;  // tubes with and without the fused transfer and scan
+    bool any_fused2 = false;                     // tubes whose equilibrium sweep is fused too
     const WTube* tube1 = nullptr;                // host copy of the descriptor when ntubes == 1
 };
 
@@ -371,6 +376,9 @@ static int launch_rgf_kernels(const RgfLayout& y, const WLaunch& lp, hipStream_t
     }
     if (y.any_fused)
         hipLaunchKernelGGL(transfer_scan_kernel, dim3(2 * y.nsblk), dim3(64 * FUSED_WAVES), 0, st, lp, y.nsblk);
+    if (y.any_fused2)
+        hipLaunchKernelGGL((rgf_eq_fused_kernel<NEGF_SUB_K, NEGF_SUB_NS>), dim3(2 * y.nsblk), dim3(64 * EQF_WAVES), 0,
+                           st, lp, y.nsblk);
     for (int ns = 1; ns <= NEGF_SUB_NS; ++ns) {
         if (y.fast_n[ns] == 0) continue;
         WLaunch lf = lp;
@@ -469,6 +477,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     y.red_dst_off = d_red_dst_off_.as<long long>();
     y.dec = dec_;
     y.any_fused = any_fused_;
+    y.any_fused2 = any_fused2_;
     y.any_unfused = any_unfused_;
     if (ntubes_ == 1 && tube0_.size() == sizeof(WTube)) y.tube1 = reinterpret_cast<const WTube*>(tube0_.data());
     const WLaunch lp = launch_params(y, d_diagG, d_chain_out, tube_active, d_site_out);
@@ -661,7 +670,7 @@ static int scratch_address(void** out)
 // RgfPlan::build gives), chain groups, the row-reduction tables, the tube
 // descriptor, and the status word cleared.
 __global__ void one_tube_tables_kernel(WTube tb, WTube* tubes, int items, int P, int nfull, int fast, int* item_tube,
-                                       int* item_list, int G, int* sblk_tube, int* sblk_chain, int nE,
+                                       int* item_list, int G, int rows, int* sblk_tube, int* sblk_chain, int nE,
                                        int* chain_tube, long long* red_src_off, int* red_nrows, int* red_N,
                                        long long* red_dst_off, int* status)
 {
@@ -682,7 +691,7 @@ __global__ void one_tube_tables_kernel(WTube tb, WTube* tubes, int items, int P,
     if (i0 == 0) {
         *tubes = tb;
         red_src_off[0] = 0;
-        red_nrows[0] = G;
+        red_nrows[0] = rows;
         red_N[0] = tb.N;
         red_dst_off[0] = 0;
         *status = 0;
@@ -744,6 +753,10 @@ int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d
     }
     const size_t N = (size_t)p->N, E = (size_t)nE;
     const size_t G = (E + 63) / 64, items = G * (size_t)P;
+    // the same launch forms as RgfPlan::build
+    WTube tb = make_wtube(one_tube(p, d_U, nE), P, L, 0, 0);
+    if (tb.fused && !d_diagG && L == NEGF_SEG_MAX && p->N % L == 0) tb.fused = 2;
+    const size_t rows = tb.fused == 2 ? 2 * G : G;
     size_t off = 0;
     auto take = [&off](size_t bytes) {
         const size_t o = off;
@@ -754,7 +767,7 @@ int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d
     const size_t o_it = take(items * sizeof(int)), o_il = take(items * sizeof(int));
     const size_t o_ct = take(E * sizeof(int)), o_gt = take(G * sizeof(int)), o_gc = take(G * sizeof(int));
     const size_t o_Tm = take(items * 9 * 64 * sizeof(double)), o_St = take(items * 10 * 64 * sizeof(double));
-    const size_t o_Dd = take(E * 3 * sizeof(double)), o_pt = take(G * N * sizeof(double));
+    const size_t o_Dd = take(E * 3 * sizeof(double)), o_pt = take(rows * N * sizeof(double));
     const size_t o_r1 = take(sizeof(long long)), o_r2 = take(sizeof(int)), o_r3 = take(sizeof(int));
     const size_t o_r4 = take(sizeof(long long)), o_stat = take(sizeof(int));
     if (off > NEGF_SCRATCH_BYTES) {
@@ -798,7 +811,9 @@ int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d
     // unless diag G is written
     const int nfull = (p->N % L == 0) ? P : P - 1;
     const bool fast = !y.write_G;
-    if (fast) {
+    if (tb.fused == 2) {
+        // no separate sweep
+    } else if (fast) {
         y.fast_off[L / NEGF_SUB_K] = 0;
         y.fast_n[L / NEGF_SUB_K] = (int)G * nfull;
         y.slow_off = (int)G * nfull;
@@ -807,16 +822,16 @@ int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d
         y.slow_off = 0;
         y.slow_n = (int)items;
     }
-    const WTube tb = make_wtube(one_tube(p, d_U, nE), P, L, 0, 0);
-    y.any_fused = tb.fused != 0;
-    y.any_unfused = !y.any_fused;
+    y.any_fused = tb.fused == 1;
+    y.any_fused2 = tb.fused == 2;
+    y.any_unfused = tb.fused == 0;
     y.tube1 = &tb;
 
     hipLaunchKernelGGL(pack_eq_chains, dim3((nE + 255) / 256), dim3(256), 0, st, nE, (const double2*)d_E,
                        (const double2*)d_coef, p->t_even, y.chains);
     const int nt = (int)std::min<size_t>(1024, (std::max(items, E) + 255) / 256);
     hipLaunchKernelGGL(one_tube_tables_kernel, dim3(nt), dim3(256), 0, st, tb, d_tubes, (int)items, P, nfull,
-                       fast ? 1 : 0, const_cast<int*>(y.item_tube), const_cast<int*>(y.item_list), (int)G,
+                       fast ? 1 : 0, const_cast<int*>(y.item_tube), const_cast<int*>(y.item_list), (int)G, (int)rows,
                        const_cast<int*>(y.sblk_tube), const_cast<int*>(y.sblk_chain), nE,
                        const_cast<int*>(y.chain_tube), const_cast<long long*>(y.red_src_off),
                        const_cast<int*>(y.red_nrows), const_cast<int*>(y.red_N),

@@ -99,6 +99,7 @@ class RgfPlan {
     bool write_G_ = false;
     int nitems_ = 0, nchains_ = 0, ntubes_ = 0, nsblk_ = 0;
     bool any_fused_ = false, any_unfused_ = false;  // tubes with and without the fused transfer and scan
+    bool any_fused2_ = false;                        // tubes whose equilibrium sweep is fused too
     std::vector<unsigned char> tube0_;               // kernel descriptor of the first tube (bytes)
     int max_N_ = 0, max_P_ = 0, max_groups_ = 0;
     ChainDesc* d_chains_ = nullptr;

@@ -59,7 +59,8 @@ struct WTube {
     int sample[3];        // non-equilibrium integrand sample sites (-1: none)
     int ext;              // index of the tube in its caller (the active mask)
     const double* Ucp;    // device contact potentials (2 values), or null: Uc0/Uc1
-    int fused;            // transfer matrices and scan in one launch (transfer_scan_kernel)
+    int fused;            // 1: transfer matrices and walks in one launch (transfer_scan_kernel);
+                          // 2: and the equilibrium sweep too (rgf_eq_fused_kernel, two rows per group)
 };
 
 struct WLaunch {
@@ -367,9 +368,21 @@ __device__ __forceinline__ ItemCtx item_ctx(const WLaunch& lp, int item, int lan
 // The wave's segment of U is staged once in LDS with coalesced loads; the
 // per-site reads are then wave-uniform LDS broadcasts instead of a vector
 // load (and its wait) per site.
+// All loads are issued before the first LDS write (clamped addresses, so they
+// are unconditional): a loop that writes each value before loading the next
+// waits a full memory round trip per iteration.
 __device__ __forceinline__ void stage_segment(double* us, const double* __restrict__ U, int lo, int hi, int lane)
 {
-    for (int i = lane; i < hi - lo; i += 64) us[i] = U[lo + i];
+    constexpr int R = NEGF_SEG_MAX / 64;
+    const int n = hi - lo;  // wave-uniform
+    if (n > 0) {
+        double v[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[q] = U[lo + min(lane + 64 * q, n - 1)];
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (lane + 64 * q < n) us[lane + 64 * q] = v[q];
+    }
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -776,10 +789,13 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
 // of one wavefront, in chunks of FUSED_CHUNK steps whose matrices are read
 // from LDS together; the arithmetic is scan_kernel's.
 #define FUSED_CHUNK 4
-template <bool V>
+// LDS: the states go to Sl[(segment * 8 + c) * 32 + chain] (c 0-3: u_lo,
+// u_{lo-1}; 4-7: v_{hi-1}, v_hi) and the v exponent to Se[segment * 32 +
+// chain] instead of the segment-state array in memory.
+template <bool V, bool LDS = false>
 __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, const double (*sig)[32],
                                            const WTube& tube, const ChainDesc& ch, int g, int hf, int cl,
-                                           bool valid, int chain)
+                                           bool valid, int chain, double* Sl = nullptr, int* Se = nullptr)
 {
     const int P = tube.P, L = tube.L, N = tube.N;
     const double te = ch.t_even, to = tube.t_odd;
@@ -814,20 +830,24 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
         for (int q = 0; q < FUSED_CHUNK; ++q) {
             if (i0 + q < P) {  // wave-uniform
                 const int sg = V ? P - 1 - (i0 + q) : i0 + q;
-                double* S = Sb + (long long)sg * 10 * 64;
+                double* S = LDS ? Sl + (sg * 8 + (V ? 4 : 0)) * 32 + cl : Sb + (long long)sg * 10 * 64;
+                constexpr int st = LDS ? 32 : 64;
                 if (!V) {
-                    S[0 * 64] = x.x;  // (u_lo, u_{lo-1})
-                    S[1 * 64] = x.y;
-                    S[2 * 64] = y.x;
-                    S[3 * 64] = y.y;
-                    S[4 * 64] = (double)ex;
+                    S[0 * st] = x.x;  // (u_lo, u_{lo-1})
+                    S[1 * st] = x.y;
+                    S[2 * st] = y.x;
+                    S[3 * st] = y.y;
+                    if (!LDS) S[4 * st] = (double)ex;
                     mat2_apply_raw(m[q], x, y, ex);
                 } else {
-                    S[0 * 64] = y.x;  // (v_{hi-1}, v_hi)
-                    S[1 * 64] = y.y;
-                    S[2 * 64] = x.x;
-                    S[3 * 64] = x.y;
-                    S[4 * 64] = (double)ex;
+                    S[0 * st] = y.x;  // (v_{hi-1}, v_hi)
+                    S[1 * st] = y.y;
+                    S[2 * st] = x.x;
+                    S[3 * st] = x.y;
+                    if (LDS)
+                        Se[sg * 32 + cl] = ex;
+                    else
+                        S[4 * st] = (double)ex;
                     const int hi = min(N, sg * L + L);
                     mat2_apply_inv_raw(m[q], ((hi - 1) & 1) ? 1.0 : idet_odd, x, y, ex);
                 }
@@ -872,7 +892,7 @@ __global__ __launch_bounds__(64 * FUSED_WAVES, 1) void transfer_scan_kernel(WLau
     tube.nchain = __builtin_amdgcn_readfirstlane(tube.nchain);
     // (a half without chains still runs, on copies of the group's last chain:
     // the sweeps read the states of all 64 lanes and mask the contributions)
-    if (!tube.fused || tube_skipped(lp, tube)) return;       // uniform over the block
+    if (tube.fused != 1 || tube_skipped(lp, tube)) return;  // uniform over the block
     const int P = tube.P, L = tube.L, N = tube.N;
     const int h = lane >> 5, cl = lane & 31;
     const int lc = g * 64 + hf * 32 + cl;
@@ -882,12 +902,25 @@ __global__ __launch_bounds__(64 * FUSED_WAVES, 1) void transfer_scan_kernel(WLau
     const double2 E = make_double2(ch.Er, ch.Ei);
     const double te = ch.t_even, to = tube.t_odd;
 
-    // potentials of the wave's two segments (zero past the end of the tube)
-    for (int i = lane; i < 2 * L; i += 64) {
-        const int hh = i >= L ? 1 : 0;
-        const int j = i - hh * L;
-        const int site = (w + FUSED_WAVES * hh) * L + j;
-        us_all[w][hh][j] = site < N ? tube.U[site] : 0.0;
+    // potentials of the wave's two segments (zero past the end of the tube);
+    // every load is issued before the first LDS write
+    {
+        constexpr int R = 2 * NEGF_SEG_MAX / 64;
+        double uv[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = lane + 64 * q;
+            const int hh = i >= L ? 1 : 0;
+            const int site = (w + FUSED_WAVES * hh) * L + (i - hh * L);
+            uv[q] = tube.U[min(site, N - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = lane + 64 * q;
+            const int hh = i >= L ? 1 : 0;
+            const int site = (w + FUSED_WAVES * hh) * L + (i - hh * L);
+            if (i < 2 * L) us_all[w][hh][i - hh * L] = site < N ? uv[q] : 0.0;
+        }
     }
     // lead self-energies: the lanes of segment 0 (closed form, also stored for
     // the sweeps), or as the decimation launch left them in the chain table
@@ -980,7 +1013,10 @@ struct Ckpt {
 // the wave; red is this wave's LDS tile [K][65] (rows padded against bank
 // conflicts); every lane returns the sum for site (lane % K).  LDS operations of one wavefront execute in order,
 // so no barrier is needed between the writes and the reads.
-template <int K>
+// H = 2: the two halves of the wavefront (lanes 0-31, 32-63) are summed
+// separately; lane i < K holds the first half's sum of site i and lane 32 + i
+// the second half's.
+template <int K, int H = 1>
 __device__ __forceinline__ double lane_sum_transpose(const double* red, int lane)
 {
     __builtin_amdgcn_wave_barrier();
@@ -992,7 +1028,7 @@ __device__ __forceinline__ double lane_sum_transpose(const double* red, int lane
 #pragma unroll
     for (int c = 0; c < C; ++c) acc += red[i * 65 + r * C + c];
 #pragma unroll
-    for (int d = K; d < 64; d <<= 1) acc += __shfl_xor(acc, d, 64);
+    for (int d = K; d < 64 / H; d <<= 1) acc += __shfl_xor(acc, d, 64);
     __builtin_amdgcn_wave_barrier();
     return acc;
 }
@@ -1212,7 +1248,7 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 // segment.  wp and wq give W's mantissa as wp u1 v0 - wq u0 v1 in the
 // recurrence's own variables (t_lo, t_lo for the plain recurrence; Im E and
 // t_lo^2 / Im E for the scaled one, where the segment constant C cancels).
-template <int K, int NS, class KC>
+template <int K, int NS, class KC, int H = 1>
 __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, double2 ux, double2 uy, double2 x,
                                              double2 y, int e, double2 coef, double wp, double wq, double* red,
                                              int lane, double* prow)
@@ -1270,8 +1306,8 @@ __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, doub
             else
                 fstep<0>(k, us[j * K + i], ux, uy);
         }
-        const double r = lane_sum_transpose<K>(red, lane);
-        if (lane < K) prow[j * K + lane] = r;
+        const double r = lane_sum_transpose<K, H>(red, lane);
+        if ((lane & (64 / H - 1)) < K && prow) prow[j * K + (lane & (K - 1))] = r;
         if (j + 1 < NS) {
             const int d = ce[j + 1] - ce[j];
             scale2(ux, d);
@@ -1328,6 +1364,192 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
     } else {
         const Coefs k = make_coefs(E, ch.t_even, tube.t_odd);
         eq_full_body<K, NS>(k, us, ux, uy, x, y, e, coef, te, te, red, lane, prow);
+    }
+}
+
+// ------------------------------------------------------------ fused equilibrium charge
+// Equilibrium charge of tubes whose 17..32 segments are all full (N a
+// multiple of L = K NS), without diag G^R: transfer matrices, walks and the
+// sweep in one launch, the segment states kept in LDS.  One workgroup of
+// EQF_WAVES wavefronts per (64-chain group, half of 32 chains); wavefront w
+// takes segments w + EQF_WAVES q, q = 0..3, as two passes over its two halves
+// (lanes 0-31: q = 0, 2; lanes 32-63: q = 1, 3).
+//   phase 1  transfer matrices of its segments into Tl (transfer_kernel's
+//            arithmetic), the lead self-energies by the lanes of segment 0;
+//   phase 2  the u walk (wavefront 0) and the v walk (wavefront 1) over Tl,
+//            writing the states into LDS (scan_kernel's arithmetic);
+//   phase 3  the sweep of each segment (sweep_eq_full_kernel's arithmetic),
+//            both halves at once, each half's 32 contributions summed into
+//            its own partial row: the tube has two rows per chain group.
+// LDS: Tl (72 KiB) is dead after phase 2 and holds the sweep's transpose
+// buffers in phase 3; the states (68 KiB), the staged potentials of a pass
+// (16 KiB) and the lead self-energies (1 KiB) complete the 157 KiB.  Two
+// wavefronts per SIMD (the sweep's registers), one workgroup per CU.
+#define EQF_WAVES 8
+#define EQF_TL_BYTES (FUSED_SEGS * 9 * 32 * 8)
+#define EQF_ST_BYTES (FUSED_SEGS * 8 * 32 * 8)
+#define EQF_SE_BYTES (FUSED_SEGS * 32 * 4)
+#define EQF_US_BYTES (EQF_WAVES * 2 * NEGF_SEG_MAX * 8)
+#define EQF_LDS_BYTES (EQF_TL_BYTES + EQF_ST_BYTES + EQF_SE_BYTES + EQF_US_BYTES + 4 * 32 * 8)
+
+template <int K, int NS>
+__global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch lp, int ngroups)
+{
+    static_assert(EQF_WAVES * K * 65 * 8 <= EQF_TL_BYTES, "transpose buffers must fit in the Tl region");
+    constexpr int L = K * NS;
+    __shared__ __align__(16) unsigned char smem[EQF_LDS_BYTES];
+    double* Tl = reinterpret_cast<double*>(smem);
+    double* Sl = reinterpret_cast<double*>(smem + EQF_TL_BYTES);
+    int* Se = reinterpret_cast<int*>(smem + EQF_TL_BYTES + EQF_ST_BYTES);
+    double* us_all = reinterpret_cast<double*>(smem + EQF_TL_BYTES + EQF_ST_BYTES + EQF_SE_BYTES);
+    double(*sig)[32] =
+        reinterpret_cast<double(*)[32]>(smem + EQF_TL_BYTES + EQF_ST_BYTES + EQF_SE_BYTES + EQF_US_BYTES);
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int gi = blockIdx.x >> 1;
+    const int hf = blockIdx.x & 1;
+    if (gi >= ngroups) return;  // uniform over the block
+    int g;
+    const int t = group_tube(lp, gi, g);
+    WTube tube = tube_of(lp, t);
+    tube.N = __builtin_amdgcn_readfirstlane(tube.N);
+    tube.P = __builtin_amdgcn_readfirstlane(tube.P);
+    tube.nchain = __builtin_amdgcn_readfirstlane(tube.nchain);
+    if (tube.fused != 2 || tube_skipped(lp, tube)) return;  // uniform over the block
+    const int P = tube.P, N = tube.N;
+    const int h = lane >> 5, cl = lane & 31;
+    const int lc = g * 64 + hf * 32 + cl;
+    const bool valid = lc < tube.nchain;
+    const int chain = tube.chain0 + (valid ? lc : tube.nchain - 1);
+    const ChainDesc ch = lp.chains[chain];
+    const double2 E = make_double2(ch.Er, ch.Ei);
+    const double te = ch.t_even, to = tube.t_odd;
+    const bool scaled = __all(scaled_ok(E, te, to));
+    double* us_w = us_all + w * 2 * L;  // [half][L] of this wavefront's pass
+
+    // ---- phase 1: transfer matrices
+    for (int pass = 0; pass < 2; ++pass) {
+        {
+            constexpr int R = 2 * L / 64;
+            double uv[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int i = lane + 64 * q, hh = i / L;
+                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
+                uv[q] = tube.U[min(site, N - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int i = lane + 64 * q, hh = i / L;
+                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
+                us_w[i] = site < N ? uv[q] : 0.0;
+            }
+        }
+        if (pass == 0 && w == 0 && h == 0) {
+            double2 SL = make_double2(ch.SLr, ch.SLi), SR = make_double2(ch.SRr, ch.SRi);
+            if (!lp.dec_use) lead_sigma<false>(lp, valid ? lp.chains + chain : nullptr, ch, tube, &SL, &SR);
+            sig[0][cl] = SL.x;
+            sig[1][cl] = SL.y;
+            sig[2][cl] = SR.x;
+            sig[3][cl] = SR.y;
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int sgm = w + EQF_WAVES * (2 * pass + h);  // this lane's segment
+        const int lo = sgm * L;
+        const int n = sgm < P ? L : 0;
+        const double* us = us_w + h * L;
+        double2 m00 = make_double2(1.0, 0.0), m01 = make_double2(0.0, 0.0);
+        double2 m10 = make_double2(0.0, 0.0), m11 = make_double2(1.0, 0.0);
+        int eT = 0;
+        if (scaled) {
+            const SCoefs ks = make_scoefs(E, te, to);
+            if (n > 0) {
+                m11 = make_double2(ks.ai[1], 0.0);  // site lo - 1 is odd
+                transfer_loop<K>(ks, us, n, m00, m01, m10, m11, eT);
+                double mC;
+                int eC;
+                scale_prod(ks, 0, n - 1, mC, eC);
+                const double r0 = mC * ks.ai[1];  // site hi - 1 is odd
+                m00 = cscale(r0, m00);
+                m01 = cscale(r0, m01);
+                m10 = cscale(mC, m10);
+                m11 = cscale(mC, m11);
+                eT += eC;
+                renorm_mat(m00, m01, m10, m11, eT);
+            }
+        } else {
+            const Coefs k = make_coefs(E, te, to);
+            if (n > 0) transfer_loop<K>(k, us, n, m00, m01, m10, m11, eT);
+        }
+        if (sgm < P) {
+            double* T = Tl + sgm * 9 * 32 + cl;
+            T[0 * 32] = m00.x;
+            T[1 * 32] = m00.y;
+            T[2 * 32] = m01.x;
+            T[3 * 32] = m01.y;
+            T[4 * 32] = m10.x;
+            T[5 * 32] = m10.y;
+            T[6 * 32] = m11.x;
+            T[7 * 32] = m11.y;
+            T[8 * 32] = (double)eT;
+        }
+        __builtin_amdgcn_wave_barrier();  // us_w is restaged by the next pass
+    }
+    __syncthreads();
+
+    // ---- phase 2: the walks, states into LDS
+    if (w < 2 && lane < 32) {
+        if (w == 0)
+            fused_walk<false, true>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
+        else
+            fused_walk<true, true>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
+    }
+    __syncthreads();
+
+    // ---- phase 3: the sweeps
+    double* red = Tl + w * K * 65;
+    const double vmask = valid ? 1.0 : 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (w + EQF_WAVES * 2 * pass >= P) break;  // no segment in either half (wave-uniform)
+        {
+            constexpr int R = 2 * L / 64;
+            double uv[R];
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int i = lane + 64 * q, hh = i / L;
+                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
+                uv[q] = tube.U[min(site, N - 1)];
+            }
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int i = lane + 64 * q, hh = i / L;
+                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
+                us_w[i] = site < N ? uv[q] : 0.0;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int sgm = w + EQF_WAVES * (2 * pass + h);
+        const bool has = sgm < P;
+        const int sr = has ? sgm : P - 1;  // an idle half reads a real segment's states (masked below)
+        const double* S = Sl + sr * 8 * 32 + cl;
+        double2 ux = make_double2(S[0], S[32]), uy = make_double2(S[64], S[96]);
+        double2 x = make_double2(S[128], S[160]), y = make_double2(S[192], S[224]);
+        const int e = Se[sr * 32 + cl];
+        const double msk = has ? vmask : 0.0;
+        const double2 coef = make_double2(ch.cr * msk, ch.ci * msk);
+        double* prow = lp.part + tube.part_off + (long long)(2 * g + hf) * N + (long long)sr * L;
+        const double* us = us_w + h * L;
+        if (scaled) {
+            const SCoefs ks = make_scoefs(E, te, to);
+            uy = cscale(ks.ai[1], uy);  // site lo - 1 is odd
+            y = cscale(ks.ai[1], y);    // site hi - 1 is odd
+            eq_full_body<K, NS, SCoefs, 2>(ks, us, ux, uy, x, y, e, coef, E.y, te * te * ks.ie, red, lane,
+                                           has ? prow : nullptr);
+        } else {
+            const Coefs k = make_coefs(E, te, to);
+            eq_full_body<K, NS, Coefs, 2>(k, us, ux, uy, x, y, e, coef, te, te, red, lane, has ? prow : nullptr);
+        }
+        __builtin_amdgcn_wave_barrier();  // us_w and red are reused by the next pass
     }
 }
 

@@ -438,15 +438,17 @@ def with_fused(flag, fn):
             os.environ["NEGF_RGF_FUSED"] = old
 
 
-@pytest.mark.parametrize("N,L", [(2100, 0), (4096, 0), (1000, 48), (777, 32)])
+@pytest.mark.parametrize("N,L", [(2100, 0), (4096, 0), (3072, 0), (2176, 0), (1000, 48), (777, 32)])
 def test_fused_transfer_scan_matches_separate_launches(N, L):
     """Tubes of 17..32 segments take one launch for transfer matrices and
-    scan (matrices kept in LDS); it must give what the separate launches give
-    for every launch kind: charge with and without diag G^R, lead self-energies
-    in closed form and by decimation, non-equilibrium charge with and without
-    integrand samples, DOS and transmission.  Tolerance 1e-13 relative: the v
-    walk applies the adjugate as one matrix-vector product, which may round
-    differently from the separate scan's subtraction form."""
+    walks (matrices kept in LDS), and when every segment is a full 128-site
+    one the equilibrium charge without diag G^R takes one launch for the
+    sweep too (states kept in LDS, 24 and 17 segments leave some wavefronts'
+    halves idle); both must give what the separate launches give for every
+    launch kind: charge with and without diag G^R, lead self-energies in
+    closed form and by decimation, non-equilibrium charge with and without
+    integrand samples, DOS and transmission.  Tolerance 1e-13 relative: the
+    partial rows of the fused sweep sum 32 chains each instead of 64."""
     lib = _lib.load()
     U = smooth_potential(N, seed=N + 1, lo=-0.3, hi=0.3)
     E = energies(40, 40, seed=N)
