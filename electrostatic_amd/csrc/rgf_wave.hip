@@ -489,6 +489,29 @@ __device__ __forceinline__ void transfer_loop(const KC& k, UP us, int n, double2
     }
 }
 
+// Two independent full segments of L = K NS sites stepped together (twice the
+// independent FMA chains of one lane), each renormalised every K sites.
+struct Mat2x {
+    double2 m00, m01, m10, m11;
+    int e;
+};
+template <int K, int NS, class KC>
+__device__ __forceinline__ void transfer_loop2(const KC& k, const double* ua, const double* ub, Mat2x& a, Mat2x& b)
+{
+#pragma unroll 1
+    for (int q0 = 0; q0 < K * NS; q0 += K) {
+#pragma unroll
+        for (int i = 0; i < K; i += 2) {
+            mstep<0>(k, ua[q0 + i], a.m00, a.m01, a.m10, a.m11);
+            mstep<0>(k, ub[q0 + i], b.m00, b.m01, b.m10, b.m11);
+            mstep<1>(k, ua[q0 + i + 1], a.m00, a.m01, a.m10, a.m11);
+            mstep<1>(k, ub[q0 + i + 1], b.m00, b.m01, b.m10, b.m11);
+        }
+        renorm_mat(a.m00, a.m01, a.m10, a.m11, a.e);
+        renorm_mat(b.m00, b.m01, b.m10, b.m11, b.e);
+    }
+}
+
 // (256, 4): up to 4 workgroups of 4 waves per CU, enough registers for the
 // lead self-energies without spills (4, 6 or 8 waves per SIMD time the same).
 template <int K>
@@ -786,13 +809,14 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
 
 
 // One walk over the segment matrices in LDS (V: the v walk), by the 32 lanes
-// of one wavefront, in chunks of FUSED_CHUNK steps whose matrices are read
-// from LDS together; the arithmetic is scan_kernel's.
+// of one wavefront, in chunks of CH steps whose matrices are read from LDS
+// together, the state renormalised once per chunk (by exact powers of two, so
+// the chunk length changes no value); the arithmetic is scan_kernel's.
 #define FUSED_CHUNK 4
 // LDS: the states go to Sl[(segment * 8 + c) * 32 + chain] (c 0-3: u_lo,
 // u_{lo-1}; 4-7: v_{hi-1}, v_hi) and the v exponent to Se[segment * 32 +
 // chain] instead of the segment-state array in memory.
-template <bool V, bool LDS = false>
+template <bool V, bool LDS = false, int CH = FUSED_CHUNK>
 __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, const double (*sig)[32],
                                            const WTube& tube, const ChainDesc& ch, int g, int hf, int cl,
                                            bool valid, int chain, double* Sl = nullptr, int* Se = nullptr)
@@ -814,10 +838,10 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
     const double idet_odd = te / to;  // 1/det T_s when hi - 1 is even
     const long long item0 = tube.item0 + (long long)g * P;
     double* Sb = lp.St + item0 * 10 * 64 + (V ? 5 : 0) * 64 + hf * 32 + cl;
-    for (int i0 = 0; i0 < P; i0 += FUSED_CHUNK) {
-        Mat2 m[FUSED_CHUNK];
+    for (int i0 = 0; i0 < P; i0 += CH) {
+        Mat2 m[CH];
 #pragma unroll
-        for (int q = 0; q < FUSED_CHUNK; ++q) {
+        for (int q = 0; q < CH; ++q) {
             const int i = min(i0 + q, P - 1);
             const double* T = Tl + (V ? P - 1 - i : i) * 9 * 32 + cl;
             m[q].a = make_double2(T[0 * 32], T[1 * 32]);
@@ -827,7 +851,7 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
             m[q].e = T[8 * 32];
         }
 #pragma unroll
-        for (int q = 0; q < FUSED_CHUNK; ++q) {
+        for (int q = 0; q < CH; ++q) {
             if (i0 + q < P) {  // wave-uniform
                 const int sg = V ? P - 1 - (i0 + q) : i0 + q;
                 double* S = LDS ? Sl + (sg * 8 + (V ? 4 : 0)) * 32 + cl : Sb + (long long)sg * 10 * 64;
@@ -851,7 +875,7 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
                     const int hi = min(N, sg * L + L);
                     mat2_apply_inv_raw(m[q], ((hi - 1) & 1) ? 1.0 : idet_odd, x, y, ex);
                 }
-                if (q % SCAN_RENORM == SCAN_RENORM - 1) renorm(x, y, ex);
+                if (q == CH - 1) renorm(x, y, ex);
             }
         }
     }
@@ -1427,25 +1451,27 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     const bool scaled = __all(scaled_ok(E, te, to));
     double* us_w = us_all + w * 2 * L;  // [half][L] of this wavefront's pass
 
-    // ---- phase 1: transfer matrices
-    for (int pass = 0; pass < 2; ++pass) {
-        {
-            constexpr int R = 2 * L / 64;
-            double uv[R];
+    // ---- phase 1: transfer matrices.  Each lane steps its two segments
+    // (pass 0 and pass 1) together; their potentials are staged in the
+    // segments' own Tl slots (a wavefront reads them there before it writes
+    // the matrices over them, and no other wavefront touches those slots).
+    {
+        const int sa = w + EQF_WAVES * h, sb = w + EQF_WAVES * (2 + h);  // this lane's segments
+        constexpr int R = 4 * L / 64;
+        double uv[R];
 #pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int i = lane + 64 * q, hh = i / L;
-                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
-                uv[q] = tube.U[min(site, N - 1)];
-            }
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int i = lane + 64 * q, hh = i / L;
-                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
-                us_w[i] = site < N ? uv[q] : 0.0;
-            }
+        for (int q = 0; q < R; ++q) {
+            const int i = lane + 64 * q, k4 = i / L;  // k4 = 2 pass + half
+            const int site = (w + EQF_WAVES * k4) * L + (i - k4 * L);
+            uv[q] = tube.U[min(site, N - 1)];
         }
-        if (pass == 0 && w == 0 && h == 0) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = lane + 64 * q, k4 = i / L;
+            const int sg = w + EQF_WAVES * k4;
+            if (sg < P) Tl[sg * 9 * 32 + (i - k4 * L)] = uv[q];
+        }
+        if (w == 0 && h == 0) {
             double2 SL = make_double2(ch.SLr, ch.SLi), SR = make_double2(ch.SRr, ch.SRi);
             if (!lp.dec_use) lead_sigma<false>(lp, valid ? lp.chains + chain : nullptr, ch, tube, &SL, &SR);
             sig[0][cl] = SL.x;
@@ -1454,55 +1480,64 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
             sig[3][cl] = SR.y;
         }
         __builtin_amdgcn_wave_barrier();
-        const int sgm = w + EQF_WAVES * (2 * pass + h);  // this lane's segment
-        const int lo = sgm * L;
-        const int n = sgm < P ? L : 0;
-        const double* us = us_w + h * L;
-        double2 m00 = make_double2(1.0, 0.0), m01 = make_double2(0.0, 0.0);
-        double2 m10 = make_double2(0.0, 0.0), m11 = make_double2(1.0, 0.0);
-        int eT = 0;
+        // a segment past the end (P < 32) runs on segment 0's potentials; its result is not stored
+        const double* ua = Tl + (sa < P ? sa : 0) * 9 * 32;
+        const double* ub = Tl + (sb < P ? sb : 0) * 9 * 32;
+        Mat2x A, B;
+        A.m00 = B.m00 = make_double2(1.0, 0.0);
+        A.m01 = B.m01 = make_double2(0.0, 0.0);
+        A.m10 = B.m10 = make_double2(0.0, 0.0);
+        A.m11 = B.m11 = make_double2(1.0, 0.0);
+        A.e = B.e = 0;
         if (scaled) {
             const SCoefs ks = make_scoefs(E, te, to);
-            if (n > 0) {
-                m11 = make_double2(ks.ai[1], 0.0);  // site lo - 1 is odd
-                transfer_loop<K>(ks, us, n, m00, m01, m10, m11, eT);
-                double mC;
-                int eC;
-                scale_prod(ks, 0, n - 1, mC, eC);
-                const double r0 = mC * ks.ai[1];  // site hi - 1 is odd
-                m00 = cscale(r0, m00);
-                m01 = cscale(r0, m01);
-                m10 = cscale(mC, m10);
-                m11 = cscale(mC, m11);
-                eT += eC;
-                renorm_mat(m00, m01, m10, m11, eT);
+            A.m11 = B.m11 = make_double2(ks.ai[1], 0.0);  // site lo - 1 is odd
+            transfer_loop2<K, NS>(ks, ua, ub, A, B);
+            double mC;
+            int eC;
+            scale_prod(ks, 0, L - 1, mC, eC);
+            const double r0 = mC * ks.ai[1];  // site hi - 1 is odd
+            Mat2x* M[2] = {&A, &B};
+#pragma unroll
+            for (int z = 0; z < 2; ++z) {
+                M[z]->m00 = cscale(r0, M[z]->m00);
+                M[z]->m01 = cscale(r0, M[z]->m01);
+                M[z]->m10 = cscale(mC, M[z]->m10);
+                M[z]->m11 = cscale(mC, M[z]->m11);
+                M[z]->e += eC;
+                renorm_mat(M[z]->m00, M[z]->m01, M[z]->m10, M[z]->m11, M[z]->e);
             }
         } else {
             const Coefs k = make_coefs(E, te, to);
-            if (n > 0) transfer_loop<K>(k, us, n, m00, m01, m10, m11, eT);
+            transfer_loop2<K, NS>(k, ua, ub, A, B);
         }
-        if (sgm < P) {
-            double* T = Tl + sgm * 9 * 32 + cl;
-            T[0 * 32] = m00.x;
-            T[1 * 32] = m00.y;
-            T[2 * 32] = m01.x;
-            T[3 * 32] = m01.y;
-            T[4 * 32] = m10.x;
-            T[5 * 32] = m10.y;
-            T[6 * 32] = m11.x;
-            T[7 * 32] = m11.y;
-            T[8 * 32] = (double)eT;
+        __builtin_amdgcn_wave_barrier();  // every lane has read its potentials
+        const Mat2x* M[2] = {&A, &B};
+        const int sg2[2] = {sa, sb};
+#pragma unroll
+        for (int z = 0; z < 2; ++z) {
+            if (sg2[z] < P) {
+                double* T = Tl + sg2[z] * 9 * 32 + cl;
+                T[0 * 32] = M[z]->m00.x;
+                T[1 * 32] = M[z]->m00.y;
+                T[2 * 32] = M[z]->m01.x;
+                T[3 * 32] = M[z]->m01.y;
+                T[4 * 32] = M[z]->m10.x;
+                T[5 * 32] = M[z]->m10.y;
+                T[6 * 32] = M[z]->m11.x;
+                T[7 * 32] = M[z]->m11.y;
+                T[8 * 32] = (double)M[z]->e;
+            }
         }
-        __builtin_amdgcn_wave_barrier();  // us_w is restaged by the next pass
     }
     __syncthreads();
 
     // ---- phase 2: the walks, states into LDS
     if (w < 2 && lane < 32) {
         if (w == 0)
-            fused_walk<false, true>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
+            fused_walk<false, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
         else
-            fused_walk<true, true>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
+            fused_walk<true, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
     }
     __syncthreads();
 
