@@ -1395,9 +1395,9 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
 // Equilibrium charge of tubes whose 17..32 segments are all full (N a
 // multiple of L = K NS), without diag G^R: transfer matrices, walks and the
 // sweep in one launch, the segment states kept in LDS.  One workgroup of
-// EQF_WAVES wavefronts per (64-chain group, half of 32 chains); wavefront w
-// takes segments w + EQF_WAVES q, q = 0..3, as two passes over its two halves
-// (lanes 0-31: q = 0, 2; lanes 32-63: q = 1, 3).
+// EQF_WAVES wavefronts per (64-chain group, half of 32 chains); in phase 1
+// wavefront w takes segments w + EQF_WAVES q, q = 0..3 (lanes 0-31: q = 0, 2;
+// lanes 32-63: q = 1, 3), in phase 3 work items from a counter.
 //   phase 1  transfer matrices of its segments into Tl (transfer_kernel's
 //            arithmetic), the lead self-energies by the lanes of segment 0;
 //   phase 2  the u walk (wavefront 0) and the v walk (wavefront 1) over Tl,
@@ -1428,6 +1428,7 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     double* us_all = reinterpret_cast<double*>(smem + EQF_TL_BYTES + EQF_ST_BYTES + EQF_SE_BYTES);
     double(*sig)[32] =
         reinterpret_cast<double(*)[32]>(smem + EQF_TL_BYTES + EQF_ST_BYTES + EQF_SE_BYTES + EQF_US_BYTES);
+    __shared__ int next_item;  // phase 3 work counter
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gi = blockIdx.x >> 1;
@@ -1471,6 +1472,7 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
             const int sg = w + EQF_WAVES * k4;
             if (sg < P) Tl[sg * 9 * 32 + (i - k4 * L)] = uv[q];
         }
+        if (threadIdx.x == 0) next_item = 0;
         if (w == 0 && h == 0) {
             double2 SL = make_double2(ch.SLr, ch.SLi), SR = make_double2(ch.SRr, ch.SRi);
             if (!lp.dec_use) lead_sigma<false>(lp, valid ? lp.chains + chain : nullptr, ch, tube, &SL, &SR);
@@ -1541,29 +1543,36 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     }
     __syncthreads();
 
-    // ---- phase 3: the sweeps
+    // ---- phase 3: the sweeps.  Work item k (k < 16) is segment k in the
+    // first half of a wavefront and segment k + 16 in the second; wavefronts
+    // take items from a counter, so the older wavefronts of a SIMD, which the
+    // hardware issues first, take more of them and none is left alone long.
     double* red = Tl + w * K * 65;
     const double vmask = valid ? 1.0 : 0.0;
-    for (int pass = 0; pass < 2; ++pass) {
-        if (w + EQF_WAVES * 2 * pass >= P) break;  // no segment in either half (wave-uniform)
+    const int nwork = min(P, FUSED_SEGS / 2);
+    for (;;) {
+        int item = 0;
+        if (lane == 0) item = atomicAdd(&next_item, 1);
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= nwork) break;  // wave-uniform
         {
             constexpr int R = 2 * L / 64;
             double uv[R];
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int i = lane + 64 * q, hh = i / L;
-                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
+                const int site = (item + (FUSED_SEGS / 2) * hh) * L + (i - hh * L);
                 uv[q] = tube.U[min(site, N - 1)];
             }
 #pragma unroll
             for (int q = 0; q < R; ++q) {
                 const int i = lane + 64 * q, hh = i / L;
-                const int site = (w + EQF_WAVES * (2 * pass + hh)) * L + (i - hh * L);
+                const int site = (item + (FUSED_SEGS / 2) * hh) * L + (i - hh * L);
                 us_w[i] = site < N ? uv[q] : 0.0;
             }
         }
         __builtin_amdgcn_wave_barrier();
-        const int sgm = w + EQF_WAVES * (2 * pass + h);
+        const int sgm = item + (FUSED_SEGS / 2) * h;
         const bool has = sgm < P;
         const int sr = has ? sgm : P - 1;  // an idle half reads a real segment's states (masked below)
         const double* S = Sl + sr * 8 * 32 + cl;
@@ -1584,7 +1593,7 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
             const Coefs k = make_coefs(E, te, to);
             eq_full_body<K, NS, Coefs, 2>(k, us, ux, uy, x, y, e, coef, te, te, red, lane, has ? prow : nullptr);
         }
-        __builtin_amdgcn_wave_barrier();  // us_w and red are reused by the next pass
+        __builtin_amdgcn_wave_barrier();  // us_w and red are reused by the next item
     }
 }
 
