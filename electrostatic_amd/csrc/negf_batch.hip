@@ -36,11 +36,12 @@ class Batch {
     ~Batch()
     {
         drop_graph();
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
-        for (hipEvent_t e : {ev0, ev1, ev_in, ev_out})
+        for (hipStream_t* sp : {&stream, &stream_neq})
+            if (*sp) {
+                (void)hipStreamSynchronize(*sp);
+                (void)hipStreamDestroy(*sp);
+            }
+        for (hipEvent_t e : {ev0, ev1, ev_in, ev_out, ev_fork, ev_join})
             if (e) (void)hipEventDestroy(e);
     }
 
@@ -68,6 +69,9 @@ class Batch {
         NEGF_HIP_TRY(hipEventCreate(&ev1));
         NEGF_HIP_TRY(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
         NEGF_HIP_TRY(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+        NEGF_HIP_TRY(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        NEGF_HIP_TRY(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+        NEGF_HIP_TRY(hipStreamCreate(&stream_neq));
         int rc;
         const long long Nt = off[n];
         if ((rc = d_rho0.ensure(sizeof(double) * Nt))) return rc;
@@ -133,11 +137,21 @@ class Batch {
     }
 
     int enqueue(double* dQ) { return enqueue_on(dQ, stream); }
+    // The equilibrium and non-equilibrium plans are independent: the second
+    // runs on its own stream (forked from st and joined back before the
+    // combine, a form that stream capture records too), so the latency-bound
+    // walks of one overlap the other's sweeps.
     int enqueue_on(double* dQ, hipStream_t st, const int* tube_active = nullptr)
     {
         int rc;
+        if (n_neq > 0) {
+            NEGF_HIP_TRY(hipEventRecord(ev_fork, st));
+            NEGF_HIP_TRY(hipStreamWaitEvent(stream_neq, ev_fork, 0));
+            if ((rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, stream_neq, tube_active))) return rc;
+            NEGF_HIP_TRY(hipEventRecord(ev_join, stream_neq));
+        }
         if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st, tube_active))) return rc;
-        if (n_neq > 0 && (rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, st, tube_active))) return rc;
+        if (n_neq > 0) NEGF_HIP_TRY(hipStreamWaitEvent(st, ev_join, 0));
         const long long Nt = off.back();
         hipLaunchKernelGGL(batch_combine_kernel, dim3((unsigned)((Nt + 255) / 256)), dim3(256), 0, st, Nt,
                            d_eq.as<double>(), d_neq.as<double>(), d_rho0.as<double>(), dQ);
@@ -374,8 +388,9 @@ class Batch {
     double energy_sites = 0.0;
     RgfPlan eq_plan, neq_plan;
     DevBuf eq_chains, neq_chains, d_rho0, d_eq, d_neq;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr;
+    hipStream_t stream = nullptr, stream_neq = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_in = nullptr, ev_out = nullptr, ev_fork = nullptr,
+               ev_join = nullptr;
     bool timing = false, use_graph = false, graph_ok = false;
     hipGraphExec_t gexec = nullptr;
     double* graph_dQ = nullptr;
