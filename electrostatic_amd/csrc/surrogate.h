@@ -16,7 +16,7 @@ constexpr int SG_C = 64;      // hidden channels
 constexpr int SG_CP = 32;     // padded channel count of the input and output layers
 constexpr int SG_LAYERS = 6;  // conv layers: 3 -> 64, 4 x 64 -> 64, 64 -> 1
 constexpr int SG_IN = 3;      // real input channels: U / 0.5 eV, edge distance, per-sample feature
-constexpr int SG_TILE = 64;   // sites per workgroup
+constexpr int SG_TILE = 256;  // sites per workgroup of the convolution kernel
 
 __host__ __device__ constexpr int sg_cin(int l) { return l == 0 ? SG_CP : SG_C; }
 __host__ __device__ constexpr int sg_cout(int l) { return l == SG_LAYERS - 1 ? SG_CP : SG_C; }

@@ -17,9 +17,9 @@
 // forward convolution.
 //
 // Every convolution is an implicit GEMM on v_mfma_f32_16x16x32_bf16:
-// M = 16 output channels per wavefront, N = 16 sites per MFMA tile (4 tiles
-// = 64 sites per workgroup), K = 9 taps x input channels, with the site tile
-// and a 4-site halo on each side staged in LDS.  Operand layout: lane l
+// M = 16 output channels per MFMA (two per wavefront), N = 16 sites per MFMA
+// tile (16 tiles = 256 sites per workgroup), K = 9 taps x input channels, with
+// the site tile and a 4-site halo on each side staged in LDS.  Operand layout: lane l
 // holds A[i][k] with i = l % 16 and B[k][j] with j = l % 16 for the same
 // eight k of its 16-lane group, and receives D[i][j] for i = 4 (l / 16) + v,
 // v = 0..3, j = l % 16.  (Which eight k a lane group holds does not matter:
@@ -45,81 +45,141 @@ __device__ __forceinline__ f4 mfma_bf16(const bf16x8& a, const bf16x8& b, const 
     return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
-// Y = epilogue(conv9(X, W) [+ bias]) for one sample (blockIdx.y) and 64 sites.
-// X bf16 [B][N][CIN], W bf16 [COUT][9][CIN]; COUT/16 wavefronts.
+// Y = epilogue(conv9(X, W) [+ bias]) for one sample (blockIdx.y) and
+// SG_TILE sites, as an implicit GEMM on v_mfma_f32_16x16x32_bf16
+// (A = weights [co][ci], B = staged inputs [ci][site], D [co][site]).
+// X bf16 [B][N][CIN], W bf16 [COUT][9][CIN]; COUT/32 wavefronts, each owning
+// 32 output channels whose 9 x CIN/32 x 2 weight fragments stay in its
+// registers across the tile's SG_TILE/16 site tiles, so that every input
+// fragment read from LDS feeds two MFMAs (one per 16-channel block) and the
+// weights are read once per workgroup.
 //   EPI_TANH:       Y bf16 [B][N][COUT] = tanh(acc + b)
 //   EPI_LINEAR_OUT: Yf float [B][N] = acc(co = 0) (+ b[0] if bias)
 //   EPI_DTANH:      Y bf16 [B][N][COUT] = acc * (1 - xp^2), xp bf16 [B][N][COUT]
+// tanh for a result rounded to BF16 (relative error below 1e-5, a
+// thousandth of a BF16 ulp): tanh|x| = (1 - e) / (1 + e) with e = exp(-2|x|)
+// from v_exp_f32 (2^y) and v_rcp_f32, and the odd Taylor series
+// x - x^3/3 + 2x^5/15 below |x| = 1/16, where 1 - e cancels (truncation
+// 17/315 x^7, below 4e-9 relative there).
+__device__ __forceinline__ float sg_tanh(float x)
+{
+    const float a = fabsf(x);
+    const float e = __builtin_amdgcn_exp2f(-2.8853900817779268f * a);  // -2 / ln 2
+    const float big = (1.0f - e) * __builtin_amdgcn_rcpf(1.0f + e);
+    const float x2 = a * a;
+    const float small = a * (1.0f - x2 * (1.0f / 3.0f - x2 * (2.0f / 15.0f)));
+    return copysignf(a < 0.0625f ? small : big, x);
+}
+
+template <int EPI>
+__device__ __forceinline__ void sg_epilogue(const f4& acc, int cb, size_t site, int COUT, const float* bias,
+                                            __bf16* Y, float* Yf, const __bf16* xp)
+{
+    if (EPI == EPI_TANH) {
+        __attribute__((aligned(8))) __bf16 o[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (__bf16)sg_tanh(acc[v] + bias[cb + v]);
+        *reinterpret_cast<uint2*>(Y + site * COUT + cb) = *reinterpret_cast<const uint2*>(o);
+    } else if (EPI == EPI_LINEAR_OUT) {
+        if (cb == 0) Yf[site] = acc[0] + (bias ? bias[0] : 0.0f);
+    } else {
+        const uint2 raw = *reinterpret_cast<const uint2*>(xp + site * COUT + cb);
+        const __bf16* xq = reinterpret_cast<const __bf16*>(&raw);
+        __attribute__((aligned(8))) __bf16 o[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float y = (float)xq[v];
+            o[v] = (__bf16)(acc[v] * (1.0f - y * y));
+        }
+        *reinterpret_cast<uint2*>(Y + site * COUT + cb) = *reinterpret_cast<const uint2*>(o);
+    }
+}
+
 template <int CIN, int COUT, int EPI>
-__global__ __launch_bounds__(COUT / 16 * 64) void sg_conv_kernel(int N, const __bf16* __restrict__ X,
+__global__ __launch_bounds__(COUT / 32 * 64) void sg_conv_kernel(int N, const __bf16* __restrict__ X,
                                                                  const __bf16* __restrict__ W,
                                                                  const float* __restrict__ bias,
                                                                  __bf16* __restrict__ Y, float* __restrict__ Yf,
                                                                  const __bf16* __restrict__ xp)
 {
+    constexpr int NT = COUT / 32 * 64;  // threads
     constexpr int ROWS = SG_TILE + 2 * SG_HALO;
     constexpr int RS = CIN + 8;  // LDS row stride (elements): rows start 16 B apart in bank order
+    constexpr int V8 = CIN / 8;
+    constexpr int KC = CIN / 32;  // 32-channel k blocks
     __shared__ __attribute__((aligned(16))) __bf16 xs[ROWS * RS];
     const int b = blockIdx.y;
     const int n0 = blockIdx.x * SG_TILE;
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
-    constexpr int V8 = CIN / 8;
+    const int i = lane & 15, g = lane >> 4;
+    // stage rows n0 - 4 .. n0 + SG_TILE + 3 of the sample, eight loads in flight per thread
     const __bf16* xb = X + (size_t)b * N * CIN;
-    for (int q = threadIdx.x; q < ROWS * V8; q += COUT / 16 * 64) {
-        const int r = q / V8, c8 = q - r * V8;
-        const int n = n0 - SG_HALO + r;
-        bf16x8 v;
-        if (n >= 0 && n < N) {
-            v = *reinterpret_cast<const bf16x8*>(xb + (size_t)n * CIN + c8 * 8);
-        } else {
+    constexpr int TOT = ROWS * V8, PER = (TOT + NT - 1) / NT, CH = 8;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+    for (int q0 = 0; q0 < PER; q0 += CH) {
+        bf16x8 v[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int idx = threadIdx.x + (q0 + q) * NT;
+            const int r = idx / V8, c8 = idx - r * V8;
+            const int n = min(max(n0 - SG_HALO + r, 0), N - 1);  // clamped; rows outside the sample are zeroed below
+            v[q] = *reinterpret_cast<const bf16x8*>(xb + (size_t)n * CIN + c8 * 8);
         }
-        *reinterpret_cast<bf16x8*>(xs + r * RS + c8 * 8) = v;
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int idx = threadIdx.x + (q0 + q) * NT;
+            if (q0 + q < PER && idx < TOT) {
+                const int r = idx / V8, c8 = idx - r * V8;
+                const int n = n0 - SG_HALO + r;
+                if (n < 0 || n >= N) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[q][e] = (__bf16)0.0f;
+                }
+                *reinterpret_cast<bf16x8*>(xs + r * RS + c8 * 8) = v[q];
+            }
+        }
+    }
+    // this wavefront's weight fragments (after the staging, whose buffers are dead now)
+    bf16x8 a[2][SG_KW][KC];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const __bf16* wrow = W + (size_t)(w * 32 + h * 16 + i) * SG_KW * CIN + 8 * g;
+#pragma unroll
+        for (int tap = 0; tap < SG_KW; ++tap)
+#pragma unroll
+            for (int kc = 0; kc < KC; ++kc) a[h][tap][kc] = *reinterpret_cast<const bf16x8*>(wrow + tap * CIN + kc * 32);
     }
     __syncthreads();
-    const int i = lane & 15, g = lane >> 4;
-    const __bf16* wrow = W + (size_t)(w * 16 + i) * SG_KW * CIN + 8 * g;
-    f4 acc[4];
+    // two site tiles at a time: four independent accumulators per lane
+#pragma unroll 1
+    for (int t0 = 0; t0 < SG_TILE / 16; t0 += 2) {
+        f4 acc[2][2];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-    for (int tap = 0; tap < SG_KW; ++tap) {
+            for (int u = 0; u < 2; ++u) acc[h][u] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int c0 = 0; c0 < CIN; c0 += 32) {
-            const bf16x8 a = *reinterpret_cast<const bf16x8*>(wrow + tap * CIN + c0);
+        for (int tap = 0; tap < SG_KW; ++tap) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                // site n0 + 16 t + i + tap - 4 is staged row 16 t + i + tap
-                const bf16x8 bb = *reinterpret_cast<const bf16x8*>(xs + (16 * t + i + tap) * RS + c0 + 8 * g);
-                acc[t] = mfma_bf16(a, bb, acc[t]);
+            for (int kc = 0; kc < KC; ++kc) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    // site n0 + 16 (t0 + u) + i + tap - 4 is staged row 16 (t0 + u) + i + tap
+                    const bf16x8 bb =
+                        *reinterpret_cast<const bf16x8*>(xs + (16 * (t0 + u) + i + tap) * RS + kc * 32 + 8 * g);
+                    acc[0][u] = mfma_bf16(a[0][tap][kc], bb, acc[0][u]);
+                    acc[1][u] = mfma_bf16(a[1][tap][kc], bb, acc[1][u]);
+                }
             }
         }
-    }
-    const int cb = w * 16 + 4 * g;  // output channels cb .. cb + 3 of this lane
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const int n = n0 + 16 * t + i;
-        if (n >= N) continue;
-        const size_t site = (size_t)b * N + n;
-        if (EPI == EPI_TANH) {
-            __attribute__((aligned(8))) __bf16 o[4];
+        for (int u = 0; u < 2; ++u) {
+            const int n = n0 + 16 * (t0 + u) + i;
+            if (n >= N) continue;
+            const size_t site = (size_t)b * N + n;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) o[v] = (__bf16)tanhf(acc[t][v] + bias[cb + v]);
-            *reinterpret_cast<uint2*>(Y + site * COUT + cb) = *reinterpret_cast<const uint2*>(o);
-        } else if (EPI == EPI_LINEAR_OUT) {
-            if (cb == 0) Yf[site] = acc[t][0] + (bias ? bias[0] : 0.0f);
-        } else {
-            const uint2 raw = *reinterpret_cast<const uint2*>(xp + site * COUT + cb);
-            const __bf16* xq = reinterpret_cast<const __bf16*>(&raw);
-            __attribute__((aligned(8))) __bf16 o[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const float y = (float)xq[v];
-                o[v] = (__bf16)(acc[t][v] * (1.0f - y * y));
-            }
-            *reinterpret_cast<uint2*>(Y + site * COUT + cb) = *reinterpret_cast<const uint2*>(o);
+            for (int h = 0; h < 2; ++h) sg_epilogue<EPI>(acc[h][u], w * 32 + h * 16 + 4 * g, site, COUT, bias, Y, Yf, xp);
         }
     }
 }
@@ -404,7 +464,7 @@ static void conv(int N, int B, const __bf16* X, const __bf16* W, const float* bi
                  const __bf16* xp, hipStream_t s)
 {
     dim3 g((N + SG_TILE - 1) / SG_TILE, B);
-    hipLaunchKernelGGL((sg_conv_kernel<CIN, COUT, EPI>), g, dim3(COUT / 16 * 64), 0, s, N, X, W, bias, Y, Yf, xp);
+    hipLaunchKernelGGL((sg_conv_kernel<CIN, COUT, EPI>), g, dim3(COUT / 32 * 64), 0, s, N, X, W, bias, Y, Yf, xp);
 }
 
 int Surrogate::forward(int B, const double* dU, const double* dfeat, hipStream_t s)
