@@ -513,6 +513,10 @@ int RgfPlan::check_status(hipStream_t st, int* status)
     NEGF_HIP_TRY(hipStreamSynchronize(st));
     if (h) NEGF_HIP_TRY(hipMemsetAsync(d_status_.p, 0, sizeof(int), st));
     *status = h;
+    if (h & 2) {  // bit 1: set by rgf_eq_fused_kernel when a wait exceeded its bound
+        set_error("fused RGF kernel: a wavefront's wait for the v walk exceeded its bound");
+        return NEGF_EDEVICE;
+    }
     return NEGF_OK;
 }
 

@@ -815,11 +815,14 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
 #define FUSED_CHUNK 4
 // LDS: the states go to Sl[(segment * 8 + c) * 32 + chain] (c 0-3: u_lo,
 // u_{lo-1}; 4-7: v_{hi-1}, v_hi) and the v exponent to Se[segment * 32 +
-// chain] instead of the segment-state array in memory.
+// chain] instead of the segment-state array in memory; with vprog the v walk
+// also publishes, after each chunk, how many segments (from the right end)
+// have their states stored.
 template <bool V, bool LDS = false, int CH = FUSED_CHUNK>
 __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, const double (*sig)[32],
                                            const WTube& tube, const ChainDesc& ch, int g, int hf, int cl,
-                                           bool valid, int chain, double* Sl = nullptr, int* Se = nullptr)
+                                           bool valid, int chain, double* Sl = nullptr, int* Se = nullptr,
+                                           int* vprog = nullptr)
 {
     const int P = tube.P, L = tube.L, N = tube.N;
     const double te = ch.t_even, to = tube.t_odd;
@@ -878,6 +881,11 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
                 if (q == CH - 1) renorm(x, y, ex);
             }
         }
+        // v walk into LDS: publish the number of segments whose states are
+        // stored (release: the state stores above are visible to a wavefront
+        // that reads the count with acquire)
+        if (V && LDS && vprog)
+            __hip_atomic_store(vprog, min(P, i0 + CH), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     if (!V && valid) {
         // the u walk ended at the end of the chain: (u_N, u_{N-1})
@@ -1272,14 +1280,12 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 // segment.  wp and wq give W's mantissa as wp u1 v0 - wq u0 v1 in the
 // recurrence's own variables (t_lo, t_lo for the plain recurrence; Im E and
 // t_lo^2 / Im E for the scaled one, where the segment constant C cancels).
-template <int K, int NS, class KC, int H = 1>
-__device__ __forceinline__ void eq_full_body(const KC& k, const double* us, double2 ux, double2 uy, double2 x,
-                                             double2 y, int e, double2 coef, double wp, double wq, double* red,
-                                             int lane, double* prow)
+// Pass 2 of a full-segment equilibrium sweep: the v checkpoints (x, y, e) at
+// the right end of every sub-block, from the segment's right-end state.
+template <int K, int NS, class KC>
+__device__ __forceinline__ void eq_checkpoints(const KC& k, const double* us, double2 x, double2 y, int e,
+                                               double2 (&cx)[NS], double2 (&cy)[NS], int (&ce)[NS])
 {
-    // pass 2: v checkpoints at the right end of every sub-block
-    double2 cx[NS], cy[NS];
-    int ce[NS];
     cx[NS - 1] = x;
     cy[NS - 1] = y;
     ce[NS - 1] = e;
@@ -1295,8 +1301,17 @@ __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, doub
         cy[j - 1] = y;
         ce[j - 1] = e;
     }
+}
 
-    // pass 3
+// Pass 3: per sub-block, v recomputed from its checkpoint, then the forward
+// u steps with the contributions, summed over the lanes of each half (H) into
+// the partial row prow.
+template <int K, int NS, class KC, int H = 1>
+__device__ __forceinline__ void eq_sweep_pass3(const KC& k, const double* us, double2 ux, double2 uy,
+                                               const double2 (&cx)[NS], const double2 (&cy)[NS],
+                                               const int (&ce)[NS], double2 coef, double wp, double wq,
+                                               double* red, int lane, double* prow)
+{
 #pragma unroll
     for (int j = 0; j < NS; ++j) {
         double2 vr[K];
@@ -1338,6 +1353,17 @@ __device__ __forceinline__ void eq_full_body(const KC& k, const double* us, doub
             scale2(uy, d);
         }
     }
+}
+
+template <int K, int NS, class KC, int H = 1>
+__device__ __forceinline__ void eq_full_body(const KC& k, const double* us, double2 ux, double2 uy, double2 x,
+                                             double2 y, int e, double2 coef, double wp, double wq, double* red,
+                                             int lane, double* prow)
+{
+    double2 cx[NS], cy[NS];
+    int ce[NS];
+    eq_checkpoints<K, NS>(k, us, x, y, e, cx, cy, ce);
+    eq_sweep_pass3<K, NS, KC, H>(k, us, ux, uy, cx, cy, ce, coef, wp, wq, red, lane, prow);
 }
 
 // Equilibrium sweep of a full segment of NS sub-blocks (the common case),
@@ -1401,15 +1427,20 @@ __global__ __launch_bounds__(256, NS <= 4 ? 3 : 2) void sweep_eq_full_kernel(WLa
 //   phase 1  transfer matrices of its segments into Tl (transfer_kernel's
 //            arithmetic), the lead self-energies by the lanes of segment 0;
 //   phase 2  the u walk (wavefront 0) and the v walk (wavefront 1) over Tl,
-//            writing the states into LDS (scan_kernel's arithmetic);
+//            writing the states into LDS (scan_kernel's arithmetic); the v
+//            walk publishes its progress, and wavefronts 2.. run the
+//            checkpoint pass of one sweep item each as soon as its segments'
+//            v states are stored;
 //   phase 3  the sweep of each segment (sweep_eq_full_kernel's arithmetic),
-//            both halves at once, each half's 32 contributions summed into
-//            its own partial row: the tube has two rows per chain group.
+//            two segments per item (one per half of the wavefront), each
+//            half's 32 contributions summed into its own partial row: the tube
+//            has two rows per chain group.
 // LDS: Tl (72 KiB) is dead after phase 2 and holds the sweep's transpose
 // buffers in phase 3; the states (68 KiB), the staged potentials of a pass
 // (16 KiB) and the lead self-energies (1 KiB) complete the 157 KiB.  Two
 // wavefronts per SIMD (the sweep's registers), one workgroup per CU.
 #define EQF_WAVES 8
+#define EQF_WAIT_MAX (1 << 22)  // bound on a wavefront's polls of the v walk's progress
 #define EQF_TL_BYTES (FUSED_SEGS * 9 * 32 * 8)
 #define EQF_ST_BYTES (FUSED_SEGS * 8 * 32 * 8)
 #define EQF_SE_BYTES (FUSED_SEGS * 32 * 4)
@@ -1429,6 +1460,7 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     double(*sig)[32] =
         reinterpret_cast<double(*)[32]>(smem + EQF_TL_BYTES + EQF_ST_BYTES + EQF_SE_BYTES + EQF_US_BYTES);
     __shared__ int next_item;  // phase 3 work counter
+    __shared__ int v_done;     // segments (from the right end) whose v states the v walk has stored
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int gi = blockIdx.x >> 1;
@@ -1451,6 +1483,26 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     const double te = ch.t_even, to = tube.t_odd;
     const bool scaled = __all(scaled_ok(E, te, to));
     double* us_w = us_all + w * 2 * L;  // [half][L] of this wavefront's pass
+    const int nwork = (P + 1) / 2;      // sweep items
+    const int npre = min(EQF_WAVES - 2, nwork);
+    // segment of item j in half hh (negative: none, when P is odd)
+    auto seg_of = [P](int j, int hh) { return P - 1 - 2 * j - hh; };
+    // the potentials of item j's two segments into us_w (zeros for a missing one)
+    auto stage_item = [&](int j) {
+        constexpr int R = 2 * L / 64;
+        double uv[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = lane + 64 * q, hh = i / L;
+            const int sg = seg_of(j, hh);
+            uv[q] = tube.U[max(sg, 0) * L + (i - hh * L)];
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int i = lane + 64 * q, hh = i / L;
+            us_w[i] = seg_of(j, hh) >= 0 ? uv[q] : 0.0;
+        }
+    };
 
     // ---- phase 1: transfer matrices.  Each lane steps its two segments
     // (pass 0 and pass 1) together; their potentials are staged in the
@@ -1472,7 +1524,10 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
             const int sg = w + EQF_WAVES * k4;
             if (sg < P) Tl[sg * 9 * 32 + (i - k4 * L)] = uv[q];
         }
-        if (threadIdx.x == 0) next_item = 0;
+        if (threadIdx.x == 0) {
+            next_item = npre;
+            v_done = 0;
+        }
         if (w == 0 && h == 0) {
             double2 SL = make_double2(ch.SLr, ch.SLi), SR = make_double2(ch.SRr, ch.SRi);
             if (!lp.dec_use) lead_sigma<false>(lp, valid ? lp.chains + chain : nullptr, ch, tube, &SL, &SR);
@@ -1534,64 +1589,113 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     }
     __syncthreads();
 
-    // ---- phase 2: the walks, states into LDS
-    if (w < 2 && lane < 32) {
-        if (w == 0)
-            fused_walk<false, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
-        else
-            fused_walk<true, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
+    // ---- phase 2: the walks (wavefronts 0 and 1), states into LDS.  The
+    // other wavefronts run pass 2 (the v checkpoints) of items 0..npre-1
+    // meanwhile, each as soon as the v walk has stored the states of its two
+    // segments; the checkpoints stay in their registers for phase 3.
+    double2 cx[NS], cy[NS];
+    int ce[NS];
+    int pre = -1;  // this wavefront's item with its checkpoints already taken
+    if (w < 2) {
+        if (lane < 32) {
+            if (w == 0)
+                fused_walk<false, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
+            else
+                fused_walk<true, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se, &v_done);
+        }
+    } else if (w - 2 < npre) {
+        pre = w - 2;
+        const int need = min(P, 2 * pre + 2);  // the v walk stores segment P-2-2 pre at its step 2 pre + 1
+        for (int it = 0;; ++it) {
+            const int d = __builtin_amdgcn_readfirstlane(
+                __hip_atomic_load(&v_done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP));
+            if (d >= need) break;
+            if (it >= EQF_WAIT_MAX) {  // never expected: flag it and go on rather than wait without end
+                if (lane == 0) atomicOr(lp.status, 2);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+        stage_item(pre);
+        __builtin_amdgcn_wave_barrier();
+        const int sr = max(seg_of(pre, h), 0);
+        const double* S = Sl + sr * 8 * 32 + cl;
+        double2 x = make_double2(S[128], S[160]), y = make_double2(S[192], S[224]);
+        const int e = Se[sr * 32 + cl];
+        const double* us = us_w + h * L;
+        // opaque copies: otherwise the compiler keeps these coefficients live
+        // into phase 3 (which computes the same ones), past the register budget
+        double2 Eo = E;
+        double teo = te, too = to;
+        asm volatile("" : "+v"(Eo.x), "+v"(Eo.y), "+v"(teo), "+v"(too));
+        if (scaled) {
+            const SCoefs ks = make_scoefs(Eo, teo, too);
+            y = cscale(ks.ai[1], y);  // site hi - 1 is odd
+            eq_checkpoints<K, NS>(ks, us, x, y, e, cx, cy, ce);
+        } else {
+            const Coefs k = make_coefs(Eo, teo, too);
+            eq_checkpoints<K, NS>(k, us, x, y, e, cx, cy, ce);
+        }
     }
     __syncthreads();
 
-    // ---- phase 3: the sweeps.  Work item k (k < 16) is segment k in the
-    // first half of a wavefront and segment k + 16 in the second; wavefronts
-    // take items from a counter, so the older wavefronts of a SIMD, which the
-    // hardware issues first, take more of them and none is left alone long.
+    // ---- phase 3: the sweeps.  Work item j is segment P-1-2j in the first
+    // half of a wavefront and segment P-2-2j in the second (items from the
+    // right end, whose v states the v walk stores first); after their
+    // pre-taken items, wavefronts take items from a counter, so the older
+    // wavefronts of a SIMD, which the hardware issues first, take more of
+    // them and none is left alone long.
     double* red = Tl + w * K * 65;
     const double vmask = valid ? 1.0 : 0.0;
-    const int nwork = min(P, FUSED_SEGS / 2);
+    // pass 3 of item j (its v checkpoints in cx, cy, ce)
+    auto finish_item = [&](int j) {
+        const int sgm = seg_of(j, h);
+        const bool has = sgm >= 0;
+        const int sr = has ? sgm : 0;  // an idle half reads a real segment's states (masked below)
+        const double* S = Sl + sr * 8 * 32 + cl;
+        double2 ux = make_double2(S[0], S[32]), uy = make_double2(S[64], S[96]);
+        const double msk = has ? vmask : 0.0;
+        const double2 coef = make_double2(ch.cr * msk, ch.ci * msk);
+        double* prow = has ? lp.part + tube.part_off + (long long)(2 * g + hf) * N + (long long)sr * L : nullptr;
+        const double* us = us_w + h * L;
+        if (scaled) {
+            const SCoefs ks = make_scoefs(E, te, to);
+            uy = cscale(ks.ai[1], uy);  // site lo - 1 is odd
+            eq_sweep_pass3<K, NS, SCoefs, 2>(ks, us, ux, uy, cx, cy, ce, coef, E.y, te * te * ks.ie, red, lane,
+                                             prow);
+        } else {
+            const Coefs k = make_coefs(E, te, to);
+            eq_sweep_pass3<K, NS, Coefs, 2>(k, us, ux, uy, cx, cy, ce, coef, te, te, red, lane, prow);
+        }
+        __builtin_amdgcn_wave_barrier();  // us_w and red are reused by the next item
+    };
+    if (pre >= 0) finish_item(pre);
     for (;;) {
         int item = 0;
         if (lane == 0) item = atomicAdd(&next_item, 1);
         item = __builtin_amdgcn_readfirstlane(item);
         if (item >= nwork) break;  // wave-uniform
-        {
-            constexpr int R = 2 * L / 64;
-            double uv[R];
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int i = lane + 64 * q, hh = i / L;
-                const int site = (item + (FUSED_SEGS / 2) * hh) * L + (i - hh * L);
-                uv[q] = tube.U[min(site, N - 1)];
-            }
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int i = lane + 64 * q, hh = i / L;
-                const int site = (item + (FUSED_SEGS / 2) * hh) * L + (i - hh * L);
-                us_w[i] = site < N ? uv[q] : 0.0;
-            }
-        }
+        stage_item(item);
         __builtin_amdgcn_wave_barrier();
-        const int sgm = item + (FUSED_SEGS / 2) * h;
-        const bool has = sgm < P;
-        const int sr = has ? sgm : P - 1;  // an idle half reads a real segment's states (masked below)
+        const int sgm = seg_of(item, h);
+        const bool has = sgm >= 0;
+        const int sr = has ? sgm : 0;  // an idle half reads a real segment's states (masked below)
         const double* S = Sl + sr * 8 * 32 + cl;
         double2 ux = make_double2(S[0], S[32]), uy = make_double2(S[64], S[96]);
         double2 x = make_double2(S[128], S[160]), y = make_double2(S[192], S[224]);
         const int e = Se[sr * 32 + cl];
         const double msk = has ? vmask : 0.0;
         const double2 coef = make_double2(ch.cr * msk, ch.ci * msk);
-        double* prow = lp.part + tube.part_off + (long long)(2 * g + hf) * N + (long long)sr * L;
+        double* prow = has ? lp.part + tube.part_off + (long long)(2 * g + hf) * N + (long long)sr * L : nullptr;
         const double* us = us_w + h * L;
         if (scaled) {
             const SCoefs ks = make_scoefs(E, te, to);
             uy = cscale(ks.ai[1], uy);  // site lo - 1 is odd
             y = cscale(ks.ai[1], y);    // site hi - 1 is odd
-            eq_full_body<K, NS, SCoefs, 2>(ks, us, ux, uy, x, y, e, coef, E.y, te * te * ks.ie, red, lane,
-                                           has ? prow : nullptr);
+            eq_full_body<K, NS, SCoefs, 2>(ks, us, ux, uy, x, y, e, coef, E.y, te * te * ks.ie, red, lane, prow);
         } else {
             const Coefs k = make_coefs(E, te, to);
-            eq_full_body<K, NS, Coefs, 2>(k, us, ux, uy, x, y, e, coef, te, te, red, lane, has ? prow : nullptr);
+            eq_full_body<K, NS, Coefs, 2>(k, us, ux, uy, x, y, e, coef, te, te, red, lane, prow);
         }
         __builtin_amdgcn_wave_barrier();  // us_w and red are reused by the next item
     }
