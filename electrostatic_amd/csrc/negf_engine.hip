@@ -221,6 +221,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
     long long poff = 0;
     int items = 0;
     max_N_ = 0;
+    max_rows_ = 0;
     max_P_ = 0;
     max_groups_ = 0;
     any_fused_ = any_fused2_ = any_unfused_ = false;
@@ -269,6 +270,7 @@ int RgfPlan::build(int kind, bool write_G, const std::vector<RgfTube>& tubes, Ch
         poff += (long long)rows * tb.N;
         rdst.push_back(tb.out_off);
         max_N_ = std::max(max_N_, tb.N);
+        max_rows_ = std::max(max_rows_, rows);
         max_P_ = std::max(max_P_, P);
         max_groups_ = std::max(max_groups_, G);
     }
@@ -315,6 +317,7 @@ struct RgfLayout {
     int kind = 0;
     bool write_G = false;
     int nitems = 0, nchains = 0, ntubes = 0, nsblk = 0, max_N = 0;
+    int max_rows = 1 << 30;  // most partial rows of a tube (the reduction's form)
     ChainDesc* chains = nullptr;
     const WTube* tubes = nullptr;
     const int* item_tube = nullptr;
@@ -418,9 +421,15 @@ static int launch_rgf_finish(const RgfLayout& y, const WLaunch& lp, double* d_ou
         NEGF_HIP_TRY(hipGetLastError());
         return NEGF_OK;
     }
-    dim3 g1((y.max_N + 15) / 16, y.ntubes);
-    hipLaunchKernelGGL(reduce_rows1_kernel, g1, dim3(256), 0, st, y.part, y.red_src_off, y.red_nrows, y.red_N,
-                       d_out, y.red_dst_off, y.tubes, tube_active);
+    if (y.max_rows <= 16) {
+        dim3 g0((y.max_N + 255) / 256, y.ntubes);
+        hipLaunchKernelGGL(reduce_rows_few_kernel, g0, dim3(256), 0, st, y.part, y.red_src_off, y.red_nrows,
+                           y.red_N, d_out, y.red_dst_off, y.tubes, tube_active);
+    } else {
+        dim3 g1((y.max_N + 15) / 16, y.ntubes);
+        hipLaunchKernelGGL(reduce_rows1_kernel, g1, dim3(256), 0, st, y.part, y.red_src_off, y.red_nrows, y.red_N,
+                           d_out, y.red_dst_off, y.tubes, tube_active);
+    }
     NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }
@@ -452,6 +461,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
     y.ntubes = ntubes_;
     y.nsblk = nsblk_;
     y.max_N = max_N_;
+    y.max_rows = max_rows_;
     y.chains = d_chains_;
     y.tubes = d_tubes_.as<WTube>();
     y.item_tube = d_item_tube_.as<int>();
@@ -793,6 +803,7 @@ int negf_rgf_diag_accumulate_noalloc(const negf_chain_params* p, const double* d
     y.ntubes = 1;
     y.nsblk = (int)G;
     y.max_N = p->N;
+    y.max_rows = (int)rows;
     y.chains = reinterpret_cast<ChainDesc*>(b + o_ch);
     WTube* d_tubes = reinterpret_cast<WTube*>(b + o_tb);
     y.tubes = d_tubes;

@@ -101,7 +101,7 @@ class RgfPlan {
     bool any_fused_ = false, any_unfused_ = false;  // tubes with and without the fused transfer and scan
     bool any_fused2_ = false;                        // tubes whose equilibrium sweep is fused too
     std::vector<unsigned char> tube0_;               // kernel descriptor of the first tube (bytes)
-    int max_N_ = 0, max_P_ = 0, max_groups_ = 0;
+    int max_N_ = 0, max_P_ = 0, max_groups_ = 0, max_rows_ = 0;
     ChainDesc* d_chains_ = nullptr;
     DecParams dec_;
     std::vector<RgfTube> tubes_;

@@ -1921,6 +1921,36 @@ __global__ __launch_bounds__(256) void reduce_rows1_kernel(const double* __restr
     }
 }
 
+// The same sums when no tube has more than 16 rows (each slice of
+// reduce_rows1_kernel then holds at most one row): one thread per site adds
+// its rows in row order from 0.0, which gives the same bits, and a block
+// covers 256 sites, so a batch of short row lists is not launched as 16-site
+// blocks with idle slices.
+__global__ __launch_bounds__(256) void reduce_rows_few_kernel(const double* __restrict__ src,
+                                                              const long long* __restrict__ src_off,
+                                                              const int* __restrict__ nrows,
+                                                              const int* __restrict__ Ns, double* __restrict__ dst,
+                                                              const long long* __restrict__ dst_off,
+                                                              const WTube* __restrict__ tubes,
+                                                              const int* __restrict__ tube_active)
+{
+    const int t = blockIdx.y;
+    if (tube_active && !tube_active[tubes[t].ext]) return;  // uniform over the block
+    const int N = Ns[t];
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    const int R = nrows[t];
+    const double* p = src + src_off[t] + n;
+    double v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = r < R ? p[(long long)r * N] : 0.0;  // the loads in flight together
+    double acc = 0.0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+        if (r < R) acc += v[r];
+    dst[dst_off[t] + n] = acc;
+}
+
 // Per-chain DOS trace: sum the per-segment partials in a fixed order.
 __global__ void dos_finish_kernel(WLaunch lp, int ntubes)
 {
