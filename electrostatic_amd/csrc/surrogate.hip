@@ -104,7 +104,11 @@ __global__ __launch_bounds__(COUT / 32 * 64) void sg_conv_kernel(int N, const __
 {
     constexpr int NT = COUT / 32 * 64;  // threads
     constexpr int ROWS = SG_TILE + 2 * SG_HALO;
-    constexpr int RS = CIN + 8;  // LDS row stride (elements): rows start 16 B apart in bank order
+    // LDS rows: 64 channels (eight 16-byte chunks) are stored with chunk c of
+    // row r at chunk c ^ (r & 7), so that the 16 rows of an operand fragment
+    // read 16 distinct bank groups; narrower rows are padded by 16 bytes
+    constexpr bool SW = CIN == 64;
+    constexpr int RS = SW ? CIN : CIN + 8;  // row stride (elements)
     constexpr int V8 = CIN / 8;
     constexpr int KC = CIN / 32;  // 32-channel k blocks
     __shared__ __attribute__((aligned(16))) __bf16 xs[ROWS * RS];
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(COUT / 32 * 64) void sg_conv_kernel(int N, const __
 #pragma unroll
                     for (int e = 0; e < 8; ++e) v[q][e] = (__bf16)0.0f;
                 }
-                *reinterpret_cast<bf16x8*>(xs + r * RS + c8 * 8) = v[q];
+                *reinterpret_cast<bf16x8*>(xs + r * RS + (SW ? (c8 ^ (r & 7)) : c8) * 8) = v[q];
             }
         }
     }
@@ -166,8 +170,9 @@ __global__ __launch_bounds__(COUT / 32 * 64) void sg_conv_kernel(int N, const __
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
                     // site n0 + 16 (t0 + u) + i + tap - 4 is staged row 16 (t0 + u) + i + tap
+                    const int row = 16 * (t0 + u) + i + tap, c8 = kc * 4 + g;
                     const bf16x8 bb =
-                        *reinterpret_cast<const bf16x8*>(xs + (16 * (t0 + u) + i + tap) * RS + kc * 32 + 8 * g);
+                        *reinterpret_cast<const bf16x8*>(xs + row * RS + (SW ? (c8 ^ (row & 7)) : c8) * 8);
                     acc[0][u] = mfma_bf16(a[0][tap][kc], bb, acc[0][u]);
                     acc[1][u] = mfma_bf16(a[1][tap][kc], bb, acc[1][u]);
                 }
