@@ -1072,12 +1072,44 @@ struct SubCtx {
     double kL, kR, aL, aR;  // KIND_NONEQ / DOS weights of |v|^2 and |u|^2
 };
 
+// Row outputs of a sweep wavefront ([chain][N] arrays: diag G^R and the
+// per-site DOS).  A lane's values are staged in LDS and written as whole
+// 128-byte pieces of one chain's row (8 complex or 16 real sites, one piece
+// per 8 or 16 lanes of a store), instead of one 16- or 8-byte store per lane
+// and site into 64 different rows, whose partly written lines leave the L2
+// one by one (2.7x the G^R bytes in HBM writes).
+struct RowOut {
+    char* base = nullptr;  // row of the wavefront's first chain
+    int N = 0;             // row length (sites)
+    int nvalid = 0;        // lanes 0 .. nvalid-1 have a row
+    double* st = nullptr;  // the wavefront's LDS staging, 1024 doubles
+};
+
+// write sites [s0, s0 + n) of the staged values st[site % S][lane]
+template <class T>
+__device__ __forceinline__ void row_flush(const RowOut& o, int s0, int n, int lane)
+{
+    constexpr int S = 128 / (int)sizeof(T);  // sites per 128-byte piece (lanes per chain)
+    constexpr int CPI = 64 / S;              // chains per store
+    __builtin_amdgcn_wave_barrier();
+    const T* st = reinterpret_cast<const T*>(o.st);
+    T* base = reinterpret_cast<T*>(o.base);
+#pragma unroll 2
+    for (int r = 0; r < 64 / CPI; ++r) {  // two at a time: the sweep's registers are nearly all in use
+        const int cc = CPI * r + lane / S, sl = lane % S;
+        const T v = st[sl * 64 + cc];
+        if (sl < n && cc < o.nvalid) base[(long long)cc * o.N + s0 + sl] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 template <int K, int KIND, bool WRITE_G, bool FULL, bool SAMPLE>
 __device__ __forceinline__ void site_loop(const Coefs& k, const double (&Us)[K], const double2 (&vr)[K], double2& ux,
                                           double2& uy, const SubCtx& c, int p0, int len, bool valid,
-                                          double2* Grow, double* Srow, double* red, int lane, double& dos_acc,
+                                          const RowOut& go, const RowOut& so, double* red, int lane, double& dos_acc,
                                           const WTube& tube, const ChainDesc& ch, double* chain_out)
 {
+    static_assert(K <= 16, "one DOS row piece per sub-block");
 #pragma unroll
     for (int i = 0; i < K; ++i) {
         if (!FULL && KIND != KIND_DOS) red[i * 65 + lane] = 0.0;
@@ -1085,7 +1117,10 @@ __device__ __forceinline__ void site_loop(const Coefs& k, const double (&Us)[K],
             if (KIND == KIND_EQ) {
                 double2 g = cmul(ux, vr[i]);
                 red[i * 65 + lane] = fma(c.C.x, g.x, -c.C.y * g.y);
-                if (WRITE_G && valid) Grow[p0 + i] = cmul(c.Gs, g);
+                if (WRITE_G) {
+                    reinterpret_cast<double2*>(go.st)[(i & 7) * 64 + lane] = cmul(c.Gs, g);
+                    if ((i & 7) == 7 || (!FULL && i == len - 1)) row_flush<double2>(go, p0 + (i & ~7), (i & 7) + 1, lane);
+                }
             } else if (KIND == KIND_NONEQ) {
                 double av = cabs2(vr[i]);
                 double au = cabs2(ux);
@@ -1099,7 +1134,10 @@ __device__ __forceinline__ void site_loop(const Coefs& k, const double (&Us)[K],
             } else {
                 const double a = c.aL * cabs2(vr[i]) + c.aR * cabs2(ux);
                 dos_acc += a;
-                if (Srow && valid) Srow[p0 + i] = a;
+                if (so.base) {
+                    so.st[i * 64 + lane] = a;
+                    if (i == K - 1 || (!FULL && i == len - 1)) row_flush<double>(so, p0, i + 1, lane);
+                }
             }
             if (i & 1)
                 fwd_step<1>(k, Us[i], ux, uy);
@@ -1114,6 +1152,8 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
 {
     __shared__ double red_all[4][K * 65];
     __shared__ double us_all[4][NEGF_SEG_MAX];
+    constexpr bool ROWS = WRITE_G || KIND == KIND_DOS;  // row outputs staged in LDS
+    __shared__ double rowst_all[ROWS ? 4 : 1][ROWS ? 1024 : 1];
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int idx = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + wv);
@@ -1177,8 +1217,14 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
     double2 iW = make_double2(0.0, 0.0);  // KIND_EQ: 1 / mantissa of the segment Wronskian
     int eW = 0;                           // and its exponent
     double dos_acc = 0.0;
-    double2* Grow = WRITE_G ? (lp.diagG + (long long)ic.chain * N) : nullptr;
-    double* Srow = (KIND == KIND_DOS && lp.site_out) ? (lp.site_out + (long long)ic.chain * N) : nullptr;
+    RowOut go, so;
+    {
+        const long long row0 = (long long)(tube.chain0 + ic.g * 64) * N;
+        const int nvalid = min(64, tube.nchain - ic.g * 64);
+        if (WRITE_G) go = RowOut{reinterpret_cast<char*>(lp.diagG + row0), N, nvalid, rowst_all[ROWS ? wv : 0]};
+        if (KIND == KIND_DOS && lp.site_out)
+            so = RowOut{reinterpret_cast<char*>(lp.site_out + row0), N, nvalid, rowst_all[ROWS ? wv : 0]};
+    }
     for (int j = 0; j < nsub; ++j) {
         const int p0 = lo + j * K;
         const int len = min(K, hi - p0);
@@ -1248,13 +1294,13 @@ __global__ __launch_bounds__(256, 2) void sweep_kernel(WLaunch lp)
             for (int w = 0; w < 3; ++w) samp |= (tube.sample[w] >= p0 && tube.sample[w] < p0 + len);
         }
         if (samp)
-            site_loop<K, KIND, WRITE_G, false, true>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, Srow, red, lane,
+            site_loop<K, KIND, WRITE_G, false, true>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, go, so, red, lane,
                                                      dos_acc, tube, ch, lp.chain_out);
         else if (len == K)
-            site_loop<K, KIND, WRITE_G, true, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, Srow, red, lane,
+            site_loop<K, KIND, WRITE_G, true, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, go, so, red, lane,
                                                      dos_acc, tube, ch, lp.chain_out);
         else
-            site_loop<K, KIND, WRITE_G, false, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, Grow, Srow, red, lane,
+            site_loop<K, KIND, WRITE_G, false, false>(k, Us, vr, ux, uy, sc, p0, len, ic.valid, go, so, red, lane,
                                                       dos_acc, tube, ch, lp.chain_out);
         if (KIND != KIND_DOS) {
             double r = lane_sum_transpose<K>(red, lane);
