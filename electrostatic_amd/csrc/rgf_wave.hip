@@ -905,6 +905,100 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
     }
 }
 
+// The LDS walk of fused_walk with each step split over the two halves of the
+// wavefront (lane h * 32 + c: chain c): half 0 forms the new x, half 1 the
+// new y, each from its two matrix entries, and one v_permlane32_swap per
+// dword gives every lane both (with both operands the same register N, the
+// swap's first result is the lower half's N in every lane and its second the
+// upper half's).  u walk: N = P x + Q y with (P, Q) = (a, b) | (c, d); v walk:
+// N = (P x + Q y) / det with (P, Q) = (d, -b) | (-c, a).  A half stores the
+// value it formed itself, so the state rows 0-1 | 2-3 (u) and 6-7 | 4-5 (v)
+// come from halves 0 | 1.  Same LDS layout and progress count as fused_walk.
+__device__ __forceinline__ double swap_lo(double v, double& hi)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const auto l = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32), false, false);
+    hi = __longlong_as_double((long long)(((unsigned long long)h[1] << 32) | l[1]));
+    return __longlong_as_double((long long)(((unsigned long long)h[0] << 32) | l[0]));
+}
+
+template <bool V, int CH, int CW = 32>
+__device__ __forceinline__ void fused_walk_split(const WLaunch& lp, const double* Tl, const double (*sig)[CW],
+                                                 const WTube& tube, const ChainDesc& ch, int cl, int h, bool valid,
+                                                 int chain, double* Sl, int* Se, int* vprog = nullptr)
+{
+    const int P = tube.P, L = tube.L, N = tube.N;
+    const double te = ch.t_even, to = tube.t_odd;
+    const double tau = ((N - 1) & 1) ? to : te;  // t_{N-1}
+    const double2 SL = make_double2(sig[0][cl], sig[1][cl]);
+    const double2 SR = make_double2(sig[2][cl], sig[3][cl]);
+    double2 x, y;
+    if (!V) {
+        x = make_double2(1.0, 0.0);  // u_0
+        y = cscale(1.0 / to, SL);    // u_{-1}; t_{-1} is an odd link
+    } else {
+        x = cscale(1.0 / tau, SR);   // v_N
+        y = make_double2(1.0, 0.0);  // v_{N-1}
+    }
+    double2 own = h ? y : x;  // the component this half forms
+    int ex = 0;
+    const double idet_odd = te / to;  // 1/det T_s when hi - 1 is even
+    // this half's matrix entries (doubles of the 9-double slot): u: a b | c d; v: d b | c a
+    const int oP = V ? (h ? 4 : 6) : (h ? 4 : 0);
+    const int oQ = V ? (h ? 0 : 2) : (h ? 6 : 2);
+    const double sP = (V && h) ? -1.0 : 1.0, sQ = (V && !h) ? -1.0 : 1.0;
+    const int srow = V ? 6 - 2 * h : 2 * h;  // state rows this half stores
+    for (int i0 = 0; i0 < P; i0 += CH) {
+        double2 pm[CH], qm[CH];
+        double em[CH];
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            const int i = min(i0 + q, P - 1);
+            const double* T = Tl + (V ? P - 1 - i : i) * 9 * CW + cl;
+            pm[q] = make_double2(sP * T[oP * CW], sP * T[(oP + 1) * CW]);  // signs: exact
+            qm[q] = make_double2(sQ * T[oQ * CW], sQ * T[(oQ + 1) * CW]);
+            em[q] = T[8 * CW];
+        }
+#pragma unroll
+        for (int q = 0; q < CH; ++q) {
+            if (i0 + q < P) {  // wave-uniform
+                const int sg = V ? P - 1 - (i0 + q) : i0 + q;
+                double* S = Sl + (sg * 8 + srow) * CW + cl;
+                S[0] = own.x;
+                S[CW] = own.y;
+                if (V && h == 0) Se[sg * CW + cl] = ex;
+                double2 n = cadd(cmul(pm[q], x), cmul(qm[q], y));
+                if (V) {
+                    const int hi = min(N, sg * L + L);
+                    n = cscale(((hi - 1) & 1) ? 1.0 : idet_odd, n);
+                }
+                own = n;
+                double hx, hy;
+                x.x = swap_lo(n.x, hx);
+                x.y = swap_lo(n.y, hy);
+                y = make_double2(hx, hy);
+                ex += (int)em[q];
+                if (q == CH - 1) {
+                    renorm(x, y, ex);
+                    own = h ? y : x;
+                }
+            }
+        }
+        if (V && vprog)
+            __hip_atomic_store(vprog, min(P, i0 + CH), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (!V && valid && h == 0) {
+        // the u walk ended at the end of the chain: (u_N, u_{N-1})
+        renorm(x, y, ex);
+        const double2 Dm = csub(cscale(tau, x), cmul(SR, y));  // D = Dm 2^ex
+        double* Dp = lp.Dd + (long long)chain * 3;
+        Dp[0] = Dm.x;
+        Dp[1] = Dm.y;
+        Dp[2] = (double)ex;
+    }
+}
+
 __global__ __launch_bounds__(64 * FUSED_WAVES, 1) void transfer_scan_kernel(WLaunch lp, int ngroups)
 {
     __shared__ double Tl[FUSED_SEGS * 9 * 32];                // [segment][component][chain]
@@ -1643,12 +1737,10 @@ __global__ __launch_bounds__(64 * EQF_WAVES, 1) void rgf_eq_fused_kernel(WLaunch
     int ce[NS];
     int pre = -1;  // this wavefront's item with its checkpoints already taken
     if (w < 2) {
-        if (lane < 32) {
-            if (w == 0)
-                fused_walk<false, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se);
-            else
-                fused_walk<true, true, 8>(lp, Tl, sig, tube, ch, g, hf, cl, valid, chain, Sl, Se, &v_done);
-        }
+        if (w == 0)
+            fused_walk_split<false, 8>(lp, Tl, sig, tube, ch, cl, h, valid, chain, Sl, Se);
+        else
+            fused_walk_split<true, 8>(lp, Tl, sig, tube, ch, cl, h, valid, chain, Sl, Se, &v_done);
     } else if (w - 2 < npre) {
         pre = w - 2;
         const int need = min(P, 2 * pre + 2);  // the v walk stores segment P-2-2 pre at its step 2 pre + 1
