@@ -16,7 +16,8 @@
  * own complex struct written from the definitions (multiply = textbook,
  * divide = a*conj(b)/|b|^2, sqrt = principal branch via std::sqrt).  Parity
  * with the reference is therefore unpinned at the level of a few ulp per
- * complex operation.
+ * complex operation; the checks of the test suite are tolerances well above
+ * that level.
  */
 #ifndef NEGF_ORACLE_H_
 #define NEGF_ORACLE_H_
