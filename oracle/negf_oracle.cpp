@@ -2,8 +2,8 @@
  * negf_oracle.cpp -- CPU oracle for the NEGF charge path.
  *
  * TEST INFRASTRUCTURE ONLY (see negf_oracle.h).  A restatement of the
- * reference ELEQTRONeX algorithm, loop for loop, used as the checker for the
- * HIP product and as the timed CPU baseline.  All citations are paths in the
+ * reference ELEQTRONeX algorithm, loop for loop, against which the tests
+ * compare the HIP product, and the timed CPU baseline.  All citations are paths in the
  * reference repository.
  */
 #include "negf_oracle.h"
