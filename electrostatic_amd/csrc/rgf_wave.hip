@@ -811,18 +811,13 @@ __global__ __launch_bounds__(64 * SCAN_WAVES) void scan_kernel(WLaunch lp, int n
 // One walk over the segment matrices in LDS (V: the v walk), by the 32 lanes
 // of one wavefront, in chunks of CH steps whose matrices are read from LDS
 // together, the state renormalised once per chunk (by exact powers of two, so
-// the chunk length changes no value); the arithmetic is scan_kernel's.
+// the chunk length changes no value); the arithmetic is scan_kernel's.  The
+// states go to the segment-state array in memory, as scan_kernel writes them.
 #define FUSED_CHUNK 4
-// LDS: the states go to Sl[(segment * 8 + c) * 32 + chain] (c 0-3: u_lo,
-// u_{lo-1}; 4-7: v_{hi-1}, v_hi) and the v exponent to Se[segment * 32 +
-// chain] instead of the segment-state array in memory; with vprog the v walk
-// also publishes, after each chunk, how many segments (from the right end)
-// have their states stored.
-template <bool V, bool LDS = false, int CH = FUSED_CHUNK>
+template <bool V, int CH = FUSED_CHUNK>
 __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, const double (*sig)[32],
                                            const WTube& tube, const ChainDesc& ch, int g, int hf, int cl,
-                                           bool valid, int chain, double* Sl = nullptr, int* Se = nullptr,
-                                           int* vprog = nullptr)
+                                           bool valid, int chain)
 {
     const int P = tube.P, L = tube.L, N = tube.N;
     const double te = ch.t_even, to = tube.t_odd;
@@ -857,35 +852,26 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
         for (int q = 0; q < CH; ++q) {
             if (i0 + q < P) {  // wave-uniform
                 const int sg = V ? P - 1 - (i0 + q) : i0 + q;
-                double* S = LDS ? Sl + (sg * 8 + (V ? 4 : 0)) * 32 + cl : Sb + (long long)sg * 10 * 64;
-                constexpr int st = LDS ? 32 : 64;
+                double* S = Sb + (long long)sg * 10 * 64;
                 if (!V) {
-                    S[0 * st] = x.x;  // (u_lo, u_{lo-1})
-                    S[1 * st] = x.y;
-                    S[2 * st] = y.x;
-                    S[3 * st] = y.y;
-                    if (!LDS) S[4 * st] = (double)ex;
+                    S[0 * 64] = x.x;  // (u_lo, u_{lo-1})
+                    S[1 * 64] = x.y;
+                    S[2 * 64] = y.x;
+                    S[3 * 64] = y.y;
+                    S[4 * 64] = (double)ex;
                     mat2_apply_raw(m[q], x, y, ex);
                 } else {
-                    S[0 * st] = y.x;  // (v_{hi-1}, v_hi)
-                    S[1 * st] = y.y;
-                    S[2 * st] = x.x;
-                    S[3 * st] = x.y;
-                    if (LDS)
-                        Se[sg * 32 + cl] = ex;
-                    else
-                        S[4 * st] = (double)ex;
+                    S[0 * 64] = y.x;  // (v_{hi-1}, v_hi)
+                    S[1 * 64] = y.y;
+                    S[2 * 64] = x.x;
+                    S[3 * 64] = x.y;
+                    S[4 * 64] = (double)ex;
                     const int hi = min(N, sg * L + L);
                     mat2_apply_inv_raw(m[q], ((hi - 1) & 1) ? 1.0 : idet_odd, x, y, ex);
                 }
                 if (q == CH - 1) renorm(x, y, ex);
             }
         }
-        // v walk into LDS: publish the number of segments whose states are
-        // stored (release: the state stores above are visible to a wavefront
-        // that reads the count with acquire)
-        if (V && LDS && vprog)
-            __hip_atomic_store(vprog, min(P, i0 + CH), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     if (!V && valid) {
         // the u walk ended at the end of the chain: (u_N, u_{N-1})
@@ -905,15 +891,19 @@ __device__ __forceinline__ void fused_walk(const WLaunch& lp, const double* Tl, 
     }
 }
 
-// The LDS walk of fused_walk with each step split over the two halves of the
-// wavefront (lane h * 32 + c: chain c): half 0 forms the new x, half 1 the
+// The walk of fused_walk with the states written to LDS (Sl[(segment * 8 +
+// c) * CW + chain], c 0-3: u_lo, u_{lo-1}; 4-7: v_{hi-1}, v_hi; the v
+// exponent to Se[segment * CW + chain]), each step split over the two halves
+// of the wavefront (lane h * 32 + c: chain c): half 0 forms the new x, half 1 the
 // new y, each from its two matrix entries, and one v_permlane32_swap per
 // dword gives every lane both (with both operands the same register N, the
 // swap's first result is the lower half's N in every lane and its second the
 // upper half's).  u walk: N = P x + Q y with (P, Q) = (a, b) | (c, d); v walk:
 // N = (P x + Q y) / det with (P, Q) = (d, -b) | (-c, a).  A half stores the
 // value it formed itself, so the state rows 0-1 | 2-3 (u) and 6-7 | 4-5 (v)
-// come from halves 0 | 1.  Same LDS layout and progress count as fused_walk.
+// come from halves 0 | 1.  With vprog the v walk publishes, after each chunk,
+// how many segments (from the right end) have their states stored (release:
+// a wavefront that reads the count with acquire sees those states).
 __device__ __forceinline__ double swap_lo(double v, double& hi)
 {
     const unsigned long long b = (unsigned long long)__double_as_longlong(v);
