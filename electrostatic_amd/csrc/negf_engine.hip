@@ -1,5 +1,6 @@
 // negf_engine.hip -- host launcher for the segmented RGF kernels and the
 // Level-1 C ABI (batched RGF over an energy list for one chain).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -370,9 +371,27 @@ static WLaunch launch_params(const RgfLayout& y, double2* d_diagG, double* d_cha
     return lp;
 }
 
-// Transfer, scan and sweeps of one layout on a stream.
-static int launch_rgf_kernels(const RgfLayout& y, const WLaunch& lp, hipStream_t st)
+// true when the layout is one launch of rgf_eq_fused_kernel and nothing else
+static bool single_fused_launch(const RgfLayout& y)
 {
+    if (!y.any_fused2 || y.any_fused || y.any_unfused || y.slow_n > 0) return false;
+    for (int ns = 1; ns <= NEGF_SUB_NS; ++ns)
+        if (y.fast_n[ns] > 0) return false;
+    return true;
+}
+
+// Transfer, scan and sweeps of one layout on a stream.  With ev0/ev1 (only
+// for a single fused launch) the events are recorded by that launch itself
+// (hipExtLaunchKernel), so their interval is the kernel's own.
+static int launch_rgf_kernels(const RgfLayout& y, const WLaunch& lp, hipStream_t st, hipEvent_t ev0 = nullptr,
+                              hipEvent_t ev1 = nullptr)
+{
+    if (ev0 && single_fused_launch(y)) {
+        hipExtLaunchKernelGGL((rgf_eq_fused_kernel<NEGF_SUB_K, NEGF_SUB_NS>), dim3(2 * y.nsblk),
+                              dim3(64 * EQF_WAVES), 0, st, ev0, ev1, 0, lp, y.nsblk);
+        NEGF_HIP_TRY(hipGetLastError());
+        return NEGF_OK;
+    }
     if (y.any_unfused) {
         hipLaunchKernelGGL((transfer_kernel<NEGF_SUB_K>), dim3((y.nitems + 3) / 4), dim3(256), 0, st, lp);
         hipLaunchKernelGGL(scan_kernel, dim3(2 * y.nsblk), dim3(64 * SCAN_WAVES), 0, st, lp, y.nsblk);
@@ -505,12 +524,15 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
             ev_start_.push_back(a);
             ev_stop_.push_back(b);
         }
-        NEGF_HIP_TRY(hipEventRecord(ev_start_[ev_used_], st));
     }
-    int rc = launch_rgf_kernels(y, lp, st);
+    // a single fused launch records the events itself; otherwise they bracket the launches
+    const bool ext = timing_ && single_fused_launch(y);
+    if (timing_ && !ext) NEGF_HIP_TRY(hipEventRecord(ev_start_[ev_used_], st));
+    int rc = ext ? launch_rgf_kernels(y, lp, st, ev_start_[ev_used_], ev_stop_[ev_used_])
+                 : launch_rgf_kernels(y, lp, st);
     if (rc) return rc;
     if (timing_) {
-        NEGF_HIP_TRY(hipEventRecord(ev_stop_[ev_used_], st));
+        if (!ext) NEGF_HIP_TRY(hipEventRecord(ev_stop_[ev_used_], st));
         ++ev_used_;
     }
     return launch_rgf_finish(y, lp, d_out, st, tube_active);
