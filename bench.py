@@ -600,6 +600,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS,
             "traffic": traffic,
+            "traffic_source": ("HBM bytes per launch from rocprofv3 FETCH_SIZE and WRITE_SIZE of this "
+                               "launch, copy-calibrated: profiles/rgf_batched_8192E_4096sites_traffic.md"),
         }
 
     out = {
