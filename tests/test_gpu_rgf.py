@@ -487,3 +487,44 @@ def test_fused_transfer_scan_matches_separate_launches(N, L):
     # and against the oracle
     acc_ref, st = O.accumulate_diagG(U, BETA17, GAMMA, Uc, BETA17, GAMMA, E, coef)
     assert st == 0 and rel_err(fused["eq"], acc_ref) < 1e-10
+
+
+@pytest.mark.parametrize("N", [4096, 1000])
+def test_plan_timing_events(N):
+    """negf_rgf_plan_enable_timing / negf_rgf_plan_kernel_time: with N = 4096
+    the plan is the single fused launch, whose start and stop events the launch
+    records itself; with N = 1000 (a partial last segment) the events bracket
+    several launches.  One timed interval per execute, each positive and below
+    the wall time of the executes; the charge is the same bit for bit with and
+    without timing."""
+    import ctypes as C
+    import time
+    import torch
+    lib = _lib.load()
+    U = smooth_potential(N, seed=7)
+    E = energies(256, 256, seed=8)
+    coef = np.exp(1j * np.linspace(0, 2, E.size))
+    p = chain_params(N, BETA17, GAMMA, (U[0], U[-1]))
+    dU, dE, dc = dev_array(U), dev_array(E), dev_array(coef)
+    plan = C.c_void_p()
+    _lib.check(lib.negf_rgf_plan_create(p, ptr(dU), E.size, ptr(dE), ptr(dc), 0, C.byref(plan)))
+    try:
+        acc0, acc1 = dev_zeros(N), dev_zeros(N)
+        _lib.check(lib.negf_rgf_plan_execute(plan, ptr(acc0), None, None))
+        _lib.check(lib.negf_rgf_plan_enable_timing(plan, 1))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            _lib.check(lib.negf_rgf_plan_execute(plan, ptr(acc1), None, None))
+        torch.cuda.synchronize()
+        wall_ms = (time.perf_counter() - t0) * 1e3
+        ms, n = C.c_double(), C.c_int()
+        _lib.check(lib.negf_rgf_plan_kernel_time(plan, C.byref(ms), C.byref(n)))
+        assert n.value == 3
+        assert 0.0 < ms.value < wall_ms
+        _lib.check(lib.negf_rgf_plan_kernel_time(plan, C.byref(ms), C.byref(n)))
+        assert n.value == 0   # reading the times resets them
+        _lib.check(lib.negf_rgf_plan_status(plan, None))
+    finally:
+        lib.negf_rgf_plan_destroy(plan)
+    assert np.array_equal(to_np(acc0), to_np(acc1))
