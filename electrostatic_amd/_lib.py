@@ -194,6 +194,7 @@ SIGNATURES = {
     "negf_scf_bias_point": (C.c_int, [vp, C.c_double, dp, dp, ip, dp]),
     "negf_scf_kernel": (C.c_int, [vp, dp]),
     "negf_scf_set_external_potential": (C.c_int, [vp, dp]),
+    "negf_scf_use_graph": (C.c_int, [vp, C.c_int]),
     "negf_write_step_files": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp]),
     "negf_write_qin": (C.c_int, [C.c_char_p, C.c_int, dp, dp]),
     "negf_write_charge_components": (C.c_int, [C.c_char_p, C.c_int, dp, dp, dp, dp, dp]),

@@ -164,6 +164,14 @@ static int fused_for(int P)
 
 // Kernel-side descriptor of one tube whose items start at item0 and whose
 // partial rows start at part_off.
+int lone_tube_segment_length(int N)
+{
+    if ((N + NEGF_SEG_DEFAULT - 1) / NEGF_SEG_DEFAULT >= FUSED_MIN_SEGS) return 0;
+    const int L = (N + FUSED_SEGS - 1) / FUSED_SEGS;
+    const int Lk = (L + NEGF_SUB_K - 1) / NEGF_SUB_K * NEGF_SUB_K;
+    return (N + Lk - 1) / Lk >= FUSED_MIN_SEGS ? Lk : 0;
+}
+
 static WTube make_wtube(const RgfTube& tb, int P, int L, int item0, long long part_off)
 {
     WTube w;

@@ -89,6 +89,8 @@ class RgfPlan {
                 const int* tube_active = nullptr, double* d_site_out = nullptr);
     // Read and clear the device status word (synchronises the stream).
     int check_status(hipStream_t st, int* status);
+    // The device status word itself (an int), for a copy enqueued by the caller.
+    const int* status_ptr() const { return d_status_.as<int>(); }
     int num_items() const { return nitems_; }
     // hipEvent timing of the RGF kernels (transfer + scan + sweep)
     void enable_timing(bool on) { timing_ = on; }
@@ -123,6 +125,12 @@ void gl_path(double ar, double ai, double br, double bi, int n, int type, double
 // Segmentation of a chain of N sites into P segments of L sites (L a
 // multiple of NEGF_SUB_K, at most NEGF_SEG_MAX); 0 if N is too long.
 int choose_segments(int N, int L_req, int* P, int* L);
+// Segment length for a lone tube when none is asked for: a lone tube has one
+// chain per energy, a few chain groups, so its launch is short of wavefronts;
+// where 128-site segments would give fewer segments than the fused transfer
+// and walk launch takes, the shortest multiple of NEGF_SUB_K that keeps the
+// count within that launch's range.  0 (the default) otherwise.
+int lone_tube_segment_length(int N);
 
 // Batched solve internals shared with the batched SCF: the host part of a
 // solve (rebuilding plans when paths changed) and the bare launch sequence

@@ -42,6 +42,13 @@ class Tube {
     int prepare(int iter);
     // Enqueue one solve; Q (device, N) is ready on stream st when it returns.
     int solve(int iter, double* dQ, hipStream_t st);
+    // The launches of a solve whose prepare() has run, all on stream st and
+    // nothing on the host (capturable).
+    int enqueue_solve(double* dQ, hipStream_t st);
+    // Device status words of the plans enqueue_solve() launches (nullptr for
+    // a plan it does not launch); nonzero after a launch: check_status().
+    const int* eq_status() const { return eq_plan.status_ptr(); }
+    const int* neq_status() const { return L.noneq && n_neq > 0 ? neq_plan.status_ptr() : nullptr; }
     // Synchronise; record rootless surface GFs of the last solves.
     int check_status();
     // check_status, then NEGF_EROOT if any were recorded since the last call.

@@ -296,7 +296,7 @@ RgfTube Tube::rgf_tube(const double* dU, const double* dUc, int nchain, int seg_
     tb.chain0 = 0;
     tb.nchain = nchain;
     tb.out_off = 0;
-    tb.L = seg_len >= 0 ? seg_len : p.segment_length;
+    tb.L = seg_len >= 0 ? seg_len : p.segment_length > 0 ? p.segment_length : lone_tube_segment_length(N);
     // else-if order of the reference (NEGF_Common.cpp:3469-3495)
     tb.sample[0] = N / 2;
     tb.sample[1] = N / 4;
@@ -606,16 +606,7 @@ int Tube::solve(int iter, double* dQ, hipStream_t st)
         NEGF_HIP_TRY(hipEventDestroy(ev));
     }
     if ((rc = prepare(iter))) return rc;
-    if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, stream))) return rc;
-    const bool have_neq = L.noneq && n_neq > 0;
-    if (have_neq) {
-        if ((rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, stream))) return rc;
-    } else {
-        if ((rc = fill_device(d_neq.as<double>(), N, 0.0, stream))) return rc;
-    }
-    hipLaunchKernelGGL(combine_charge_kernel, dim3((N + 255) / 256), dim3(256), 0, stream, N, d_eq.as<double>(),
-                       d_neq.as<double>(), d_rho0.as<double>(), dQ);
-    NEGF_HIP_TRY(hipGetLastError());
+    if ((rc = enqueue_solve(dQ, stream))) return rc;
     stream_pending = true;
     if (st != stream) {
         hipEvent_t ev;
@@ -624,6 +615,22 @@ int Tube::solve(int iter, double* dQ, hipStream_t st)
         NEGF_HIP_TRY(hipStreamWaitEvent(st, ev, 0));
         NEGF_HIP_TRY(hipEventDestroy(ev));
     }
+    return NEGF_OK;
+}
+
+int Tube::enqueue_solve(double* dQ, hipStream_t st)
+{
+    int rc;
+    if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st))) return rc;
+    const bool have_neq = L.noneq && n_neq > 0;
+    if (have_neq) {
+        if ((rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, st))) return rc;
+    } else {
+        if ((rc = fill_device(d_neq.as<double>(), N, 0.0, st))) return rc;
+    }
+    hipLaunchKernelGGL(combine_charge_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, d_eq.as<double>(),
+                       d_neq.as<double>(), d_rho0.as<double>(), dQ);
+    NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }
 

@@ -267,7 +267,9 @@ struct Scf {
     ~Scf()
     {
         if (st) (void)hipStreamSynchronize(st);
-        for (DevBuf* d : {&d_K, &d_U, &d_Q, &d_Uext}) d->release();
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (DevBuf* d : {&d_K, &d_U, &d_Q, &d_Uext, &d_V, &d_step, &d_active}) d->release();
+        h_stat.release();
         if (st) (void)hipStreamDestroy(st);
     }
     Tube* tube = nullptr;
@@ -277,6 +279,15 @@ struct Scf {
     DevBuf d_K, d_U, d_Q, d_Uext;
     bool has_ext = false;
     hipStream_t st = nullptr;
+    // One iteration captured in a graph and replayed: gate voltage and
+    // Broyden step live on the device, the Broyden norms and the plans'
+    // status words come back in one pinned buffer (4 doubles, 2 ints).
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    long long g_paths = -1;
+    bool g_ext = false;
+    DevBuf d_V, d_step, d_active;
+    HostBuf h_stat;
 };
 
 // Dense stand-in for the Poisson solve + gather (not in the reference):
@@ -1062,6 +1073,17 @@ int negf_scf_create(negf_tube* t, const negf_scf_params* p, negf_scf** out)
     if (!rc) rc = s.d_U.ensure(sizeof(double) * N);
     if (!rc) rc = s.d_Q.ensure(sizeof(double) * N);
     if (!rc) rc = s.br.init(N, p->Broyden_fraction, p->Broyden_threshold_maxstep, p->Broyden_norm_type);
+    if (!rc) rc = s.d_V.ensure(sizeof(double));
+    if (!rc) rc = s.d_step.ensure(sizeof(int));
+    if (!rc) rc = s.d_active.ensure(sizeof(int));
+    if (!rc) rc = s.h_stat.ensure(4 * sizeof(double) + 2 * sizeof(int));
+    if (!rc) {
+        const int one = 1;
+        if (hipMemcpy(s.d_active.p, &one, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+            set_error("hipMemcpy of the SCF step flag failed");
+            rc = NEGF_EDEVICE;
+        }
+    }
     if (!rc) {
         // Set_Broyden_Parallel: n_in starts at the initial deposit value
         rc = s.br.reset(0, p->NS_initial_deposit_value, s.st);
@@ -1097,6 +1119,17 @@ int negf_scf_kernel(const negf_scf* s, double* K)
     return NEGF_OK;
 }
 
+int negf_scf_use_graph(negf_scf* h, int on)
+{
+    clear_error();
+    if (!h) {
+        set_error("null argument");
+        return NEGF_EARG;
+    }
+    h->s.use_graph = on != 0;
+    return NEGF_OK;
+}
+
 int negf_scf_set_external_potential(negf_scf* h, const double* U_ext)
 {
     clear_error();
@@ -1120,6 +1153,125 @@ int negf_scf_set_external_potential(negf_scf* h, const double* U_ext)
 // c_TransportSolver::Solve (Transport.cpp:359-539): iterate electrostatics ->
 // NEGF -> Broyden until Broyden_Norm <= Broyden_max_norm, then
 // Reset_ForNextBiasStep.
+// The potential of one SCF iteration written where the loop and the tube
+// read it: U_i = U_ext_i - V - sum_j K_ij n_j (the sum as in
+// coulomb_potential_kernel) into U and tU, the contact potentials U_0 and
+// U_{N-1} into tUc.  V is read from the device so that one captured
+// iteration serves every bias point.
+__global__ __launch_bounds__(256) void scf_potential_kernel(int N, const double* __restrict__ K,
+                                                            const double* __restrict__ n,
+                                                            const double* __restrict__ U_ext,
+                                                            const double* __restrict__ V, double* __restrict__ U,
+                                                            double* __restrict__ tU, double* __restrict__ tUc)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= N) return;
+    const double* __restrict__ k = K + (size_t)row * N;
+    double acc = 0.0;
+    for (int j = lane; j < N; j += 64) acc = fma(k[j], n[j], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        const double u = (U_ext ? U_ext[row] : 0.0) - V[0] - acc;
+        U[row] = u;
+        tU[row] = u;
+        if (row == 0) tUc[0] = u;
+        if (row == N - 1) tUc[1] = u;
+    }
+}
+
+static void scf_enqueue_potential(Scf& s, hipStream_t q)
+{
+    Tube& t = *s.tube;
+    hipLaunchKernelGGL(scf_potential_kernel, dim3((t.N + 3) / 4), dim3(256), 0, q, t.N, s.d_K.as<double>(),
+                       s.br.dev().n_in, s.has_ext ? s.d_Uext.as<double>() : nullptr, s.d_V.as<double>(),
+                       s.d_U.as<double>(), t.d_U.as<double>(), t.d_Uc.as<double>());
+}
+
+// Potential, solve, Broyden update (step counter on the device) and the
+// read-back of the norms and status words: nothing on the host.
+static int scf_enqueue_iteration(Scf& s, hipStream_t q)
+{
+    Tube& t = *s.tube;
+    scf_enqueue_potential(s, q);
+    NEGF_HIP_TRY(hipGetLastError());
+    int rc = t.enqueue_solve(s.d_Q.as<double>(), q);
+    if (rc) return rc;
+    hipLaunchKernelGGL(broyden_batch_kernel<false>, dim3(1), dim3(kBroydenThreads), 0, q, s.br.S, s.br.max_step,
+                       s.br.fraction, s.br.norm_type, s.d_Q.as<double>(), nullptr, s.br.dev(), (size_t)0, (size_t)0,
+                       s.d_step.as<int>(), s.d_active.as<const int>());
+    NEGF_HIP_TRY(hipGetLastError());
+    char* hs = s.h_stat.as<char>();
+    NEGF_HIP_TRY(hipMemcpyAsync(hs, s.br.d_out.p, sizeof(double) * 4, hipMemcpyDeviceToHost, q));
+    NEGF_HIP_TRY(hipMemcpyAsync(hs + 4 * sizeof(double), t.eq_status(), sizeof(int), hipMemcpyDeviceToHost, q));
+    if (t.neq_status())
+        NEGF_HIP_TRY(
+            hipMemcpyAsync(hs + 4 * sizeof(double) + sizeof(int), t.neq_status(), sizeof(int), hipMemcpyDeviceToHost, q));
+    return NEGF_OK;
+}
+
+// Capture one iteration unless the graph was captured over the tube's
+// present plans and external-potential setting.
+static int scf_ensure_graph(Scf& s)
+{
+    if (s.gexec && s.g_paths == s.tube->paths_version && s.g_ext == s.has_ext) return NEGF_OK;
+    if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
+    s.gexec = nullptr;
+    NEGF_HIP_TRY(hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal));
+    const int erc = scf_enqueue_iteration(s, s.st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s.st, &g);
+    if (erc) {
+        if (g) (void)hipGraphDestroy(g);
+        return erc;
+    }
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return NEGF_EDEVICE;
+    }
+    const hipError_t e2 = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+        s.gexec = nullptr;
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(e2));
+        return NEGF_EDEVICE;
+    }
+    s.g_paths = s.tube->paths_version;
+    s.g_ext = s.has_ext;
+    return NEGF_OK;
+}
+
+// One iteration replayed from the graph: the host part of the solve first
+// (with an adaptive correction due, the integrand is sampled at this
+// iteration's potential, enqueued and finished beforehand), then one launch
+// and one synchronisation.
+static int scf_graph_iteration(Scf& s, int it, double* norm, double* l2)
+{
+    Tube& t = *s.tube;
+    int rc;
+    if (t.p.flag_adaptive_integration_limits && it % t.p.integrand_correction_interval == 0) {
+        scf_enqueue_potential(s, s.st);
+        NEGF_HIP_TRY(hipGetLastError());
+        NEGF_HIP_TRY(hipStreamSynchronize(s.st));
+    }
+    if ((rc = t.prepare(it))) return rc;
+    if ((rc = scf_ensure_graph(s))) return rc;
+    int* hs = reinterpret_cast<int*>(s.h_stat.as<char>() + 4 * sizeof(double));
+    hs[0] = hs[1] = 0;
+    NEGF_HIP_TRY(hipGraphLaunch(s.gexec, s.st));
+    NEGF_HIP_TRY(hipStreamSynchronize(s.st));
+    if ((hs[0] | hs[1]) && (rc = t.check_status())) return rc;
+    const double* o = s.h_stat.as<double>();
+    *norm = o[0];
+    *l2 = o[1];
+    s.br.step += 1;
+    if (!std::isfinite(o[0]) || !std::isfinite(o[1])) {
+        set_error("Broyden: non-finite norm");
+        return NEGF_ENAN;
+    }
+    return NEGF_OK;
+}
+
 int negf_scf_bias_point(negf_scf* h, double V_gate, double* U_out, double* Q_out, int* iters, double* final_norm)
 {
     clear_error();
@@ -1132,11 +1284,22 @@ int negf_scf_bias_point(negf_scf* h, double V_gate, double* U_out, double* Q_out
     const int N = t.N;
     double norm = 1.0, l2 = 0.0;
     int it = 0, rc = NEGF_OK;
+    if (s.use_graph) {
+        const int step = s.br.step;
+        NEGF_HIP_TRY(hipStreamSynchronize(s.st));
+        NEGF_HIP_TRY(hipMemcpy(s.d_V.p, &V_gate, sizeof(double), hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(s.d_step.p, &step, sizeof(int), hipMemcpyHostToDevice));
+    }
     do {
         if (s.br.step > s.p.Broyden_threshold_maxstep) {
             set_error("Broyden_Step has exceeded the Broyden_Threshold_MaxStep");
             rc = NEGF_ENONCONV;
             break;
+        }
+        if (s.use_graph) {
+            if ((rc = scf_graph_iteration(s, it, &norm, &l2))) break;
+            ++it;
+            continue;
         }
         hipLaunchKernelGGL(coulomb_potential_kernel, dim3((N + 3) / 4), dim3(256), 0, s.st, N, s.d_K.as<double>(),
                            s.br.dev().n_in, s.has_ext ? s.d_Uext.as<double>() : nullptr, V_gate,

@@ -492,6 +492,9 @@ class Scf:
         self._U_ext = np.ascontiguousarray(U_ext, np.float64)
         _lib.check(self.lib.negf_scf_set_external_potential(self.h, _dp(self._U_ext)))
 
+    def use_graph(self, on=True):
+        _lib.check(self.lib.negf_scf_use_graph(self.h, int(on)))
+
     def bias_point(self, V_gate):
         N = self.tube.N
         U, Q = np.zeros(N), np.zeros(N)

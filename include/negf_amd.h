@@ -170,8 +170,9 @@ typedef struct {
                                     /* given, set from the first correction */
     int flatband_dos_integration_pts;
     double flatband_dos_integration_limits[2];  /* eV (NEGF_Common.H:417)  */
-    int segment_length;             /* RGF segment length (0: default); no */
-                                    /* reference counterpart (performance)  */
+    int segment_length;             /* RGF segment length (0: chosen from  */
+                                    /* the tube length); no reference      */
+                                    /* counterpart (performance only)      */
 } negf_tube_params;
 
 /* Fill p with the reference defaults (NEGF_Common.H:384-476, CNT.H:17-18). */
@@ -340,6 +341,12 @@ int negf_scf_bias_point(negf_scf *s, double V_gate, double *U_out,
 int negf_scf_set_external_potential(negf_scf *s, const double *U_ext);
 /* Dense kernel K (N x N, row-major, V per e) used by the stand-in. */
 int negf_scf_kernel(const negf_scf *s, double *K);
+/* No reference counterpart (performance only): on (the default), each
+ * iteration of negf_scf_bias_point replays one captured hipGraph (potential,
+ * solve, Broyden update, read-back), recaptured when the tube's paths
+ * change; off, the same launches are enqueued one by one.  Results are the
+ * same bit for bit. */
+int negf_scf_use_graph(negf_scf *s, int on);
 
 /* Batched SCF of equal tubes (no reference counterpart: one SCF loop per
  * tube there).  The tubes of batch b (same order) share one dense

@@ -256,6 +256,35 @@ def test_scf_nonequilibrium_bias_point():
     assert np.abs(U - Uo).max() < U_TOL
 
 
+@pytest.mark.parametrize("mu,adaptive", [((-1.0, -1.0), 0), ((-1.0, -1.1), 1)])
+def test_scf_graph_replay_matches_enqueued_iterations(mu, adaptive):
+    """The graph-replayed iteration (the default) and the same launches
+    enqueued one by one give the same U, Q, iteration counts and norms bit
+    for bit over a short sweep in which the external potential is switched
+    on after the first bias point.  With adaptive limits (every 3rd
+    iteration) under bias the paths are rebuilt inside the loop, so the graph
+    is recaptured there."""
+    extra = dict(flag_adaptive_integration_limits=1, integrand_correction_interval=3) if adaptive else {}
+    runs = []
+    for graph in (True, False):
+        tube, _ = pair(num_unitcells=50, mu=mu, **extra)
+        scf = negf.Scf(tube, Broyden_fraction=0.1, Broyden_max_norm=1e-6, Broyden_threshold_maxstep=200,
+                       eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
+        scf.use_graph(graph)
+        out = []
+        for k, V in enumerate((0.0, 0.3, 0.6)):
+            if k == 1:
+                scf.set_external_potential(gaussian_bumps(tube.N, seed=2, amp=0.3))
+            out.append(scf.bias_point(V))
+        runs.append(out)
+        scf.close()
+        tube.close()
+    for a, b in zip(*runs):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert a[2] == b[2] and a[3] == b[3]
+        assert a[3] <= 1e-6
+
+
 def test_current_matches_oracle():
     mu = (-1.0, -1.2)
     tube, ot = pair(num_unitcells=30, num_modes=2, mu=mu)
