@@ -43,8 +43,10 @@ class Tube {
     // Enqueue one solve; Q (device, N) is ready on stream st when it returns.
     int solve(int iter, double* dQ, hipStream_t st);
     // The launches of a solve whose prepare() has run, all on stream st and
-    // nothing on the host (capturable).
-    int enqueue_solve(double* dQ, hipStream_t st);
+    // nothing on the host (capturable).  combine = false leaves out the last
+    // step, Q = (eq + noneq) - rho0 (d_eq, d_neq, d_rho0), for a caller
+    // that fuses it into its next kernel; dQ is then unused.
+    int enqueue_solve(double* dQ, hipStream_t st, bool combine = true);
     // Device status words of the plans enqueue_solve() launches (nullptr for
     // a plan it does not launch); nonzero after a launch: check_status().
     const int* eq_status() const { return eq_plan.status_ptr(); }
@@ -87,6 +89,8 @@ class Tube {
     std::vector<double> U_host;
     double Uc_host[2] = {0, 0};
     DevBuf d_U, d_Uc, d_zero, d_rho0, d_eq, d_neq, d_Q;
+    // d_neq holds zeros (filled, and no non-equilibrium launch since)
+    bool neq_zero = false;
     DevBuf eq_chains, neq_chains, d_integ, dos_chains, dos_pairs;
     HostBuf h_stage;
     RgfPlan eq_plan, neq_plan;

@@ -275,6 +275,7 @@ int Tube::init(const negf_tube_params& prm)
     if ((rc = fill_device(d_Uc.as<double>(), 2, 0.0, stream))) return rc;
     if ((rc = fill_device(d_zero.as<double>(), std::max(N, 2), 0.0, stream))) return rc;
     if ((rc = fill_device(d_neq.as<double>(), N, 0.0, stream))) return rc;
+    neq_zero = true;
     U_host.assign(N, 0.0);
     Uc_host[0] = Uc_host[1] = 0.0;
 
@@ -482,6 +483,7 @@ int Tube::find_integrand_max()
     if (n_neq == 0) return NEGF_OK;
     int rc;
     if ((rc = fill_device(d_integ.as<double>(), 3LL * n_neq, 0.0, stream))) return rc;
+    neq_zero = false;
     if ((rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, d_integ.as<double>(), stream))) return rc;
     std::vector<double> h(3 * (size_t)n_neq);
     NEGF_HIP_TRY(hipMemcpyAsync(h.data(), d_integ.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
@@ -618,16 +620,21 @@ int Tube::solve(int iter, double* dQ, hipStream_t st)
     return NEGF_OK;
 }
 
-int Tube::enqueue_solve(double* dQ, hipStream_t st)
+int Tube::enqueue_solve(double* dQ, hipStream_t st, bool combine)
 {
     int rc;
     if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st))) return rc;
     const bool have_neq = L.noneq && n_neq > 0;
     if (have_neq) {
+        neq_zero = false;
         if ((rc = neq_plan.enqueue(d_neq.as<double>(), nullptr, nullptr, st))) return rc;
-    } else {
+    } else if (!neq_zero) {
+        // zeros once after the last non-equilibrium solve (the charge
+        // components read them)
         if ((rc = fill_device(d_neq.as<double>(), N, 0.0, st))) return rc;
+        neq_zero = true;
     }
+    if (!combine) return NEGF_OK;
     hipLaunchKernelGGL(combine_charge_kernel, dim3((N + 255) / 256), dim3(256), 0, st, N, d_eq.as<double>(),
                        d_neq.as<double>(), d_rho0.as<double>(), dQ);
     NEGF_HIP_TRY(hipGetLastError());

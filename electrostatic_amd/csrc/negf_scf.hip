@@ -34,7 +34,8 @@ struct BroydenDev {
     double* sum_vec;
     double* V;          // [max_step + 1][S]: dF / |dF|^2 per update
     double* W;          // [max_step + 1][S]
-    double* intermed;   // [max_step + 1]
+    double* intermed;   // [max_step + 1] history dots V_it . dF past the LDS copy
+    double* intermed2;  // [max_step + 1] history dots V_it . F past the LDS copy
     double* out;        // [0] max norm, [1] l2 norm, [2] |dF|^2
     double* PFp;        // previous preconditioned residual (preconditioned update only)
 };
@@ -53,17 +54,34 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// intermed[it] = sum_s Vrow(it)[s] * x[s] for it in [1, m_hi]; wave w takes
-// it = 1 + w, 1 + w + kWaves, ...
-__device__ void history_dots(const BroydenDev& b, int S, int m_hi, const double* __restrict__ x, int wave,
-                             int lane)
+// The history's dot products in one pass over V: a_it = V_it . dF and
+// b_it = V_it . F for it in [1, m_hi], each summed lane-strided and then
+// across the wavefront; wave w takes it = 1 + w, 1 + w + kWaves, ...  Entries
+// below kHistLds go to LDS, the rest to intermed (a) and intermed2 (b).
+static constexpr int kHistLds = 512;
+
+__device__ void history_dots2(const BroydenDev& b, int S, int m_hi, int wave, int lane, double* ha, double* hb)
 {
     for (int it = 1 + wave; it <= m_hi; it += kWaves) {
         const double* __restrict__ v = b.V + (size_t)it * S;
-        double acc = 0.0;
-        for (int s = lane; s < S; s += 64) acc = fma(v[s], x[s], acc);
-        acc = wave_sum(acc);
-        if (lane == 0) b.intermed[it] = acc;
+        double a = 0.0, c = 0.0;
+#pragma unroll 4
+        for (int s = lane; s < S; s += 64) {
+            const double vs = v[s];
+            a = fma(vs, b.dF[s], a);
+            c = fma(vs, b.F[s], c);
+        }
+        a = wave_sum(a);
+        c = wave_sum(c);
+        if (lane == 0) {
+            if (it < kHistLds) {
+                ha[it] = a;
+                hb[it] = c;
+            } else {
+                b.intermed[it] = a;
+                b.intermed2[it] = c;
+            }
+        }
     }
 }
 
@@ -71,12 +89,19 @@ __device__ void history_dots(const BroydenDev& b, int S, int m_hi, const double*
 // PC: the mixing term fraction * F is replaced by a preconditioned residual
 // PF = P F (P linear and fixed between history resets), and fraction * dF by
 // PF - PF_previous; everything else is the reference update.
+//
+// The reference (Broyden_Parallel_General.cpp:106-170) forms
+// sum_vec = sum_{it<=m} W_it (V_it . F) after appending V_m, W_m, which needs
+// V_it . dF first (for W_m) and a second pass over V for V_it . F.  Both dot
+// products of the old rows come from one pass here, and the new row's
+// V_m . F is summed while V_m is written; the sums over it keep the
+// reference's order (it = 1..m, the new row last).
 template <bool PC>
 __device__ void broyden_update(int S, int m, double fraction, int norm_type, const double* __restrict__ n_out,
                                const double* __restrict__ PF, const BroydenDev& b)
 {
     __shared__ double red[3][kWaves];
-    __shared__ double hist[512];
+    __shared__ double ha[kHistLds], hb[kHistLds];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
 
@@ -120,36 +145,46 @@ __device__ void broyden_update(int S, int m, double fraction, int norm_type, con
         b.out[2] = denom;
     }
 
+    double bm = 0.0;   // V_m . F
     if (m > 0) {
-        // sum_vec = W^T (V dF) over the previous updates (:106-135)
-        history_dots(b, S, m - 1, b.dF, wave, lane);
+        history_dots2(b, S, m - 1, wave, lane, ha, hb);
         __syncthreads();
-        for (int it = tid + 1; it <= m - 1 && it < 512; it += kBroydenThreads) hist[it] = b.intermed[it];
-        __syncthreads();
+        // W_m = -fraction dF + (n_in - n_prev) - sum_{it<m} W_it (V_it . dF)
+        // (:106-135); the same W_it loads give the old rows' part of sum_vec
+        double vf = 0.0;
         for (int s = tid; s < S; s += kBroydenThreads) {
-            double sv = 0.0;
-            for (int it = 1; it <= m - 1; ++it) sv = fma(b.W[(size_t)it * S + s], it < 512 ? hist[it] : b.intermed[it], sv);
+            double sa = 0.0, sb = 0.0;
+            const int mL = min(m - 1, kHistLds - 1);
+            int it = 1;
+#pragma unroll 4
+            for (; it <= mL; ++it) {
+                const double w = b.W[(size_t)it * S + s];
+                sa = fma(w, ha[it], sa);
+                sb = fma(w, hb[it], sb);
+            }
+            for (; it <= m - 1; ++it) {
+                const double w = b.W[(size_t)it * S + s];
+                sa = fma(w, b.intermed[it], sa);
+                sb = fma(w, b.intermed2[it], sb);
+            }
             const double d = b.dF[s];
             const double pd = PC ? PF[s] - b.PFp[s] : fraction * d;
-            b.V[(size_t)m * S + s] = d / denom;
-            b.W[(size_t)m * S + s] = -pd + (b.n_in[s] - b.n_prev[s]) - sv;
+            const double vm = d / denom;
+            b.V[(size_t)m * S + s] = vm;
+            b.W[(size_t)m * S + s] = -pd + (b.n_in[s] - b.n_prev[s]) - sa;
+            b.sum_vec[s] = sb;
+            vf = fma(vm, b.F[s], vf);
         }
+        vf = wave_sum(vf);
+        if (lane == 0) red[0][wave] = vf;
         __syncthreads();
-        // sum_vec = W^T (V F) including this update (:137-170)
-        history_dots(b, S, m, b.F, wave, lane);
-        __syncthreads();
-        for (int it = tid + 1; it <= m && it < 512; it += kBroydenThreads) hist[it] = b.intermed[it];
-        __syncthreads();
-        for (int s = tid; s < S; s += kBroydenThreads) {
-            double sv = 0.0;
-            for (int it = 1; it <= m; ++it) sv = fma(b.W[(size_t)it * S + s], it < 512 ? hist[it] : b.intermed[it], sv);
-            b.sum_vec[s] = sv;
-        }
+        for (int w = 0; w < kWaves; ++w) bm += red[0][w];
     }
-    // new input (:172-186), scalar = 1
+    // sum_vec (:137-170) and the new input (:172-186), scalar = 1
     for (int s = tid; s < S; s += kBroydenThreads) {
         const double prev = b.n_in[s];
-        const double sv = m > 0 ? b.sum_vec[s] : 0.0;
+        const double sv = m > 0 ? fma(b.W[(size_t)m * S + s], bm, b.sum_vec[s]) : 0.0;
+        if (m > 0) b.sum_vec[s] = sv;
         const double pf = PC ? PF[s] : fraction * b.F[s];
         if (PC) b.PFp[s] = PF[s];
         b.n_prev[s] = prev;
@@ -197,7 +232,7 @@ class Broyden {
         norm_type = norm_type_;
         int rc;
         if ((rc = d_state.ensure(sizeof(double) * 6 * (size_t)S))) return rc;
-        if ((rc = d_hist.ensure(sizeof(double) * (2 * (size_t)(max_step + 1) * S + (max_step + 1))))) return rc;
+        if ((rc = d_hist.ensure(sizeof(double) * (2 * (size_t)(max_step + 1) * S + 2 * (max_step + 1))))) return rc;
         if ((rc = d_out.ensure(sizeof(double) * 4))) return rc;
         if ((rc = d_nout.ensure(sizeof(double) * (size_t)S))) return rc;
         if ((rc = h_out.ensure(sizeof(double) * 4))) return rc;
@@ -217,6 +252,7 @@ class Broyden {
         b.V = h;
         b.W = h + (size_t)(max_step + 1) * S;
         b.intermed = h + 2 * (size_t)(max_step + 1) * S;
+        b.intermed2 = b.intermed + (max_step + 1);
         b.out = d_out.as<double>();
         b.PFp = nullptr;
         return b;
@@ -228,7 +264,7 @@ class Broyden {
         int rc;
         if (!keep_n_in && (rc = fill_device(b.n_in, S, init_value, st))) return rc;
         if ((rc = fill_device(b.n_prev, 5LL * S, 0.0, st))) return rc;
-        if ((rc = fill_device(d_hist.as<double>(), 2LL * (max_step + 1) * S + (max_step + 1), 0.0, st))) return rc;
+        if ((rc = fill_device(d_hist.as<double>(), 2LL * (max_step + 1) * S + 2 * (max_step + 1), 0.0, st))) return rc;
         step = 1;
         return NEGF_OK;
     }
@@ -268,8 +304,8 @@ struct Scf {
     {
         if (st) (void)hipStreamSynchronize(st);
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        for (DevBuf* d : {&d_K, &d_U, &d_Q, &d_Uext, &d_V, &d_step, &d_active}) d->release();
-        h_stat.release();
+        for (DevBuf* d : {&d_K, &d_U, &d_Q, &d_Uext, &d_V, &d_step}) d->release();
+        if (h_back) (void)hipHostFree(h_back);
         if (st) (void)hipStreamDestroy(st);
     }
     Tube* tube = nullptr;
@@ -280,14 +316,21 @@ struct Scf {
     bool has_ext = false;
     hipStream_t st = nullptr;
     // One iteration captured in a graph and replayed: gate voltage and
-    // Broyden step live on the device, the Broyden norms and the plans'
-    // status words come back in one pinned buffer (4 doubles, 2 ints).
+    // Broyden step live on the device; the Broyden kernel writes the norms
+    // and the plans' status words straight into mapped, coherent host memory
+    // (ScfBack), so the replay holds no copy.
     bool use_graph = true;
     hipGraphExec_t gexec = nullptr;
     long long g_paths = -1;
     bool g_ext = false;
-    DevBuf d_V, d_step, d_active;
-    HostBuf h_stat;
+    DevBuf d_V, d_step;
+    void* h_back = nullptr;    // host address
+    void* d_back = nullptr;    // the same memory as the device sees it
+};
+
+struct ScfBack {
+    double norm, l2;
+    int status[2];
 };
 
 // Dense stand-in for the Poisson solve + gather (not in the reference):
@@ -439,6 +482,7 @@ __global__ __launch_bounds__(kBroydenThreads) void broyden_batch_kernel(int S, i
     b.V = base.V + t * hist_stride;
     b.W = base.W + t * hist_stride;
     b.intermed = base.intermed + t * hist_stride;
+    b.intermed2 = base.intermed2 + t * hist_stride;
     b.out = base.out + (size_t)t * 4;
     b.PFp = PC ? base.PFp + t * state_stride : nullptr;
     if (m > max_step) {
@@ -548,7 +592,7 @@ class BatchScf {
         coulomb_kernel(tubes[0]->t.ptd, p, K);
         M = p.Broyden_threshold_maxstep;
         state_stride = 7 * (size_t)N;
-        hist_stride = 2 * (size_t)(M + 1) * N + (M + 1);
+        hist_stride = 2 * (size_t)(M + 1) * N + 2 * (M + 1);
         int rc;
         NEGF_HIP_TRY(hipStreamCreate(&st));
         if ((rc = d_K.ensure(sizeof(double) * K.size()))) return rc;
@@ -594,6 +638,7 @@ class BatchScf {
         b.V = h;
         b.W = h + (size_t)(M + 1) * N;
         b.intermed = h + 2 * (size_t)(M + 1) * N;
+        b.intermed2 = b.intermed + (M + 1);
         b.out = d_out.as<double>();
         b.PFp = s0 + 6 * (size_t)N;
         return b;
@@ -934,6 +979,150 @@ class BatchScf {
     hipGraphExec_t gexec = nullptr;
 };
 
+// The potential of one SCF iteration written where the loop and the tube
+// read it: U_i = U_ext_i - V - sum_j K_ij n_j (the sum as in
+// coulomb_potential_kernel) into U and tU, the contact potentials U_0 and
+// U_{N-1} into tUc.  V is read from the device so that one captured
+// iteration serves every bias point.
+__global__ __launch_bounds__(256) void scf_potential_kernel(int N, const double* __restrict__ K,
+                                                            const double* __restrict__ n,
+                                                            const double* __restrict__ U_ext,
+                                                            const double* __restrict__ V, double* __restrict__ U,
+                                                            double* __restrict__ tU, double* __restrict__ tUc)
+{
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= N) return;
+    const double* __restrict__ k = K + (size_t)row * N;
+    double acc = 0.0;
+    for (int j = lane; j < N; j += 64) acc = fma(k[j], n[j], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) {
+        const double u = (U_ext ? U_ext[row] : 0.0) - V[0] - acc;
+        U[row] = u;
+        tU[row] = u;
+        if (row == 0) tUc[0] = u;
+        if (row == N - 1) tUc[1] = u;
+    }
+}
+
+static void scf_enqueue_potential(Scf& s, hipStream_t q)
+{
+    Tube& t = *s.tube;
+    hipLaunchKernelGGL(scf_potential_kernel, dim3((t.N + 3) / 4), dim3(256), 0, q, t.N, s.d_K.as<double>(),
+                       s.br.dev().n_in, s.has_ext ? s.d_Uext.as<double>() : nullptr, s.d_V.as<double>(),
+                       s.d_U.as<double>(), t.d_U.as<double>(), t.d_Uc.as<double>());
+}
+
+// The tube's charge Q = (eq + noneq) - rho0 (as combine_charge_kernel)
+// into n_out, then one Broyden update with the step counter on the device
+// (the host checks the step limit before each launch), then the norms and
+// the plans' status words to the host's read-back record.  Each thread
+// forms the Q entries that it reads first in the update.
+__global__ __launch_bounds__(kBroydenThreads) void broyden_scf_kernel(int S, double fraction, int norm_type,
+                                                                        const double* __restrict__ eq,
+                                                                        const double* __restrict__ neq,
+                                                                        const double* __restrict__ rho0,
+                                                                        double* n_out, BroydenDev b,
+                                                                        int* __restrict__ step,
+                                                                        const int* __restrict__ st_eq,
+                                                                        const int* __restrict__ st_neq,
+                                                                        ScfBack* __restrict__ back)
+{
+    for (int s = threadIdx.x; s < S; s += kBroydenThreads) n_out[s] = (eq[s] + (neq ? neq[s] : 0.0)) - rho0[s];
+    const int m = *step - 1;
+    broyden_update<false>(S, m, fraction, norm_type, n_out, nullptr, b);
+    if (threadIdx.x == 0) {   // b.out[0..1] were written by this thread
+        *step = m + 2;
+        back->norm = b.out[0];
+        back->l2 = b.out[1];
+        back->status[0] = *st_eq;
+        back->status[1] = st_neq ? *st_neq : 0;
+        __threadfence_system();
+    }
+}
+
+// Potential, solve, Broyden update and read-back: nothing on the host.
+static int scf_enqueue_iteration(Scf& s, hipStream_t q)
+{
+    Tube& t = *s.tube;
+    scf_enqueue_potential(s, q);
+    NEGF_HIP_TRY(hipGetLastError());
+    int rc = t.enqueue_solve(nullptr, q, false);
+    if (rc) return rc;
+    hipLaunchKernelGGL(broyden_scf_kernel, dim3(1), dim3(kBroydenThreads), 0, q, s.br.S, s.br.fraction,
+                       s.br.norm_type, t.d_eq.as<const double>(), t.neq_status() ? t.d_neq.as<const double>() : nullptr,
+                       t.d_rho0.as<const double>(), s.d_Q.as<double>(), s.br.dev(), s.d_step.as<int>(),
+                       t.eq_status(), t.neq_status(), static_cast<ScfBack*>(s.d_back));
+    NEGF_HIP_TRY(hipGetLastError());
+    return NEGF_OK;
+}
+
+// Capture one iteration unless the graph was captured over the tube's
+// present plans and external-potential setting.
+static int scf_ensure_graph(Scf& s)
+{
+    if (s.gexec && s.g_paths == s.tube->paths_version && s.g_ext == s.has_ext) return NEGF_OK;
+    if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
+    s.gexec = nullptr;
+    NEGF_HIP_TRY(hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal));
+    const int erc = scf_enqueue_iteration(s, s.st);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s.st, &g);
+    if (erc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        s.tube->neq_zero = false;   // a zero fill may have been captured and dropped
+        if (erc) return erc;
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return NEGF_EDEVICE;
+    }
+    const hipError_t e2 = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e2 != hipSuccess) {
+        s.gexec = nullptr;
+        s.tube->neq_zero = false;
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(e2));
+        return NEGF_EDEVICE;
+    }
+    s.g_paths = s.tube->paths_version;
+    s.g_ext = s.has_ext;
+    return NEGF_OK;
+}
+
+// One iteration replayed from the graph: the host part of the solve first
+// (with an adaptive correction due, the integrand is sampled at this
+// iteration's potential, enqueued and finished beforehand), then one launch
+// and one synchronisation.
+static int scf_graph_iteration(Scf& s, int it, double* norm, double* l2)
+{
+    Tube& t = *s.tube;
+    int rc;
+    if (t.p.flag_adaptive_integration_limits && it % t.p.integrand_correction_interval == 0) {
+        scf_enqueue_potential(s, s.st);
+        NEGF_HIP_TRY(hipGetLastError());
+        NEGF_HIP_TRY(hipStreamSynchronize(s.st));
+    }
+    if ((rc = t.prepare(it))) return rc;
+    if ((rc = scf_ensure_graph(s))) return rc;
+    volatile ScfBack* back = static_cast<volatile ScfBack*>(s.h_back);
+    back->norm = std::nan("");
+    back->l2 = std::nan("");
+    back->status[0] = 0;
+    back->status[1] = 0;
+    NEGF_HIP_TRY(hipGraphLaunch(s.gexec, s.st));
+    NEGF_HIP_TRY(hipStreamSynchronize(s.st));
+    if ((back->status[0] | back->status[1]) && (rc = t.check_status())) return rc;
+    const double o[2] = {back->norm, back->l2};
+    *norm = o[0];
+    *l2 = o[1];
+    s.br.step += 1;
+    if (!std::isfinite(o[0]) || !std::isfinite(o[1])) {
+        set_error("Broyden: non-finite norm");
+        return NEGF_ENAN;
+    }
+    return NEGF_OK;
+}
+
 }  // namespace negf
 
 using namespace negf;
@@ -1075,13 +1264,12 @@ int negf_scf_create(negf_tube* t, const negf_scf_params* p, negf_scf** out)
     if (!rc) rc = s.br.init(N, p->Broyden_fraction, p->Broyden_threshold_maxstep, p->Broyden_norm_type);
     if (!rc) rc = s.d_V.ensure(sizeof(double));
     if (!rc) rc = s.d_step.ensure(sizeof(int));
-    if (!rc) rc = s.d_active.ensure(sizeof(int));
-    if (!rc) rc = s.h_stat.ensure(4 * sizeof(double) + 2 * sizeof(int));
     if (!rc) {
-        const int one = 1;
-        if (hipMemcpy(s.d_active.p, &one, sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-            set_error("hipMemcpy of the SCF step flag failed");
-            rc = NEGF_EDEVICE;
+        hipError_t eh = hipHostMalloc(&s.h_back, sizeof(ScfBack), hipHostMallocMapped | hipHostMallocCoherent);
+        if (eh == hipSuccess) eh = hipHostGetDevicePointer(&s.d_back, s.h_back, 0);
+        if (eh != hipSuccess) {
+            set_error("mapped host buffer for the SCF read-back: %s", hipGetErrorString(eh));
+            rc = NEGF_EALLOC;
         }
     }
     if (!rc) {
@@ -1153,125 +1341,6 @@ int negf_scf_set_external_potential(negf_scf* h, const double* U_ext)
 // c_TransportSolver::Solve (Transport.cpp:359-539): iterate electrostatics ->
 // NEGF -> Broyden until Broyden_Norm <= Broyden_max_norm, then
 // Reset_ForNextBiasStep.
-// The potential of one SCF iteration written where the loop and the tube
-// read it: U_i = U_ext_i - V - sum_j K_ij n_j (the sum as in
-// coulomb_potential_kernel) into U and tU, the contact potentials U_0 and
-// U_{N-1} into tUc.  V is read from the device so that one captured
-// iteration serves every bias point.
-__global__ __launch_bounds__(256) void scf_potential_kernel(int N, const double* __restrict__ K,
-                                                            const double* __restrict__ n,
-                                                            const double* __restrict__ U_ext,
-                                                            const double* __restrict__ V, double* __restrict__ U,
-                                                            double* __restrict__ tU, double* __restrict__ tUc)
-{
-    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (row >= N) return;
-    const double* __restrict__ k = K + (size_t)row * N;
-    double acc = 0.0;
-    for (int j = lane; j < N; j += 64) acc = fma(k[j], n[j], acc);
-    acc = wave_sum(acc);
-    if (lane == 0) {
-        const double u = (U_ext ? U_ext[row] : 0.0) - V[0] - acc;
-        U[row] = u;
-        tU[row] = u;
-        if (row == 0) tUc[0] = u;
-        if (row == N - 1) tUc[1] = u;
-    }
-}
-
-static void scf_enqueue_potential(Scf& s, hipStream_t q)
-{
-    Tube& t = *s.tube;
-    hipLaunchKernelGGL(scf_potential_kernel, dim3((t.N + 3) / 4), dim3(256), 0, q, t.N, s.d_K.as<double>(),
-                       s.br.dev().n_in, s.has_ext ? s.d_Uext.as<double>() : nullptr, s.d_V.as<double>(),
-                       s.d_U.as<double>(), t.d_U.as<double>(), t.d_Uc.as<double>());
-}
-
-// Potential, solve, Broyden update (step counter on the device) and the
-// read-back of the norms and status words: nothing on the host.
-static int scf_enqueue_iteration(Scf& s, hipStream_t q)
-{
-    Tube& t = *s.tube;
-    scf_enqueue_potential(s, q);
-    NEGF_HIP_TRY(hipGetLastError());
-    int rc = t.enqueue_solve(s.d_Q.as<double>(), q);
-    if (rc) return rc;
-    hipLaunchKernelGGL(broyden_batch_kernel<false>, dim3(1), dim3(kBroydenThreads), 0, q, s.br.S, s.br.max_step,
-                       s.br.fraction, s.br.norm_type, s.d_Q.as<double>(), nullptr, s.br.dev(), (size_t)0, (size_t)0,
-                       s.d_step.as<int>(), s.d_active.as<const int>());
-    NEGF_HIP_TRY(hipGetLastError());
-    char* hs = s.h_stat.as<char>();
-    NEGF_HIP_TRY(hipMemcpyAsync(hs, s.br.d_out.p, sizeof(double) * 4, hipMemcpyDeviceToHost, q));
-    NEGF_HIP_TRY(hipMemcpyAsync(hs + 4 * sizeof(double), t.eq_status(), sizeof(int), hipMemcpyDeviceToHost, q));
-    if (t.neq_status())
-        NEGF_HIP_TRY(
-            hipMemcpyAsync(hs + 4 * sizeof(double) + sizeof(int), t.neq_status(), sizeof(int), hipMemcpyDeviceToHost, q));
-    return NEGF_OK;
-}
-
-// Capture one iteration unless the graph was captured over the tube's
-// present plans and external-potential setting.
-static int scf_ensure_graph(Scf& s)
-{
-    if (s.gexec && s.g_paths == s.tube->paths_version && s.g_ext == s.has_ext) return NEGF_OK;
-    if (s.gexec) (void)hipGraphExecDestroy(s.gexec);
-    s.gexec = nullptr;
-    NEGF_HIP_TRY(hipStreamBeginCapture(s.st, hipStreamCaptureModeThreadLocal));
-    const int erc = scf_enqueue_iteration(s, s.st);
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(s.st, &g);
-    if (erc) {
-        if (g) (void)hipGraphDestroy(g);
-        return erc;
-    }
-    if (e != hipSuccess) {
-        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
-        return NEGF_EDEVICE;
-    }
-    const hipError_t e2 = hipGraphInstantiate(&s.gexec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e2 != hipSuccess) {
-        s.gexec = nullptr;
-        set_error("hipGraphInstantiate: %s", hipGetErrorString(e2));
-        return NEGF_EDEVICE;
-    }
-    s.g_paths = s.tube->paths_version;
-    s.g_ext = s.has_ext;
-    return NEGF_OK;
-}
-
-// One iteration replayed from the graph: the host part of the solve first
-// (with an adaptive correction due, the integrand is sampled at this
-// iteration's potential, enqueued and finished beforehand), then one launch
-// and one synchronisation.
-static int scf_graph_iteration(Scf& s, int it, double* norm, double* l2)
-{
-    Tube& t = *s.tube;
-    int rc;
-    if (t.p.flag_adaptive_integration_limits && it % t.p.integrand_correction_interval == 0) {
-        scf_enqueue_potential(s, s.st);
-        NEGF_HIP_TRY(hipGetLastError());
-        NEGF_HIP_TRY(hipStreamSynchronize(s.st));
-    }
-    if ((rc = t.prepare(it))) return rc;
-    if ((rc = scf_ensure_graph(s))) return rc;
-    int* hs = reinterpret_cast<int*>(s.h_stat.as<char>() + 4 * sizeof(double));
-    hs[0] = hs[1] = 0;
-    NEGF_HIP_TRY(hipGraphLaunch(s.gexec, s.st));
-    NEGF_HIP_TRY(hipStreamSynchronize(s.st));
-    if ((hs[0] | hs[1]) && (rc = t.check_status())) return rc;
-    const double* o = s.h_stat.as<double>();
-    *norm = o[0];
-    *l2 = o[1];
-    s.br.step += 1;
-    if (!std::isfinite(o[0]) || !std::isfinite(o[1])) {
-        set_error("Broyden: non-finite norm");
-        return NEGF_ENAN;
-    }
-    return NEGF_OK;
-}
-
 int negf_scf_bias_point(negf_scf* h, double V_gate, double* U_out, double* Q_out, int* iters, double* final_norm)
 {
     clear_error();
