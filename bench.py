@@ -94,7 +94,8 @@ def scf_sweep(with_cpu=True):
     potential = 4 seeded Gaussians (amplitude <= 0.3 eV), metal voltage 0..1 V
     in 0.1 V steps, Broyden fraction 0.1, absolute norm <= 1e-6, warm start.
     Reports bias points per second on the GPU (whole SCF loop per bias point,
-    device-resident) and for the oracle's serial SCF on one host thread."""
+    device-resident, each iteration replayed from one captured graph) and for
+    the oracle's serial SCF on one host thread."""
     from electrostatic_amd import negf
     from electrostatic_amd.workloads import gaussian_bumps
     keys = dict(num_unitcells=100, num_modes=1, contact_mu=(-1.0, -1.0),
@@ -482,7 +483,7 @@ def main():
                     help="also time the surrogate's forward and tangent passes alone")
     ap.add_argument("--write-diag-g", action="store_true",
                     help="also materialise diag G^R ([energy][site], 512 MiB)")
-    ap.add_argument("--only", choices=["batch", "block", "surrogate-kernels", "copy-calibration"],
+    ap.add_argument("--only", choices=["batch", "block", "surrogate-kernels", "copy-calibration", "scf"],
                     help="run one secondary leg alone and print its figures (for kernel traces and "
                          "counters; --segment-length then fixes the batch leg's segment length)")
     args = ap.parse_args()
@@ -497,7 +498,8 @@ def main():
         legs = {"batch": lambda: batch_leg(dev, segment_length=args.segment_length),
                 "block": lambda: block_leg(with_cpu=False),
                 "surrogate-kernels": lambda: surrogate_kernels_leg(dev),
-                "copy-calibration": lambda: copy_calibration_leg(dev)}
+                "copy-calibration": lambda: copy_calibration_leg(dev),
+                "scf": lambda: scf_sweep(with_cpu=False)}
         print(json.dumps({"only": args.only, "result": legs[args.only]()}), flush=True)
         return
 

@@ -211,6 +211,7 @@ __global__ __launch_bounds__(256) void coulomb_potential_kernel(int N, const dou
     if (row >= N) return;
     const double* __restrict__ k = K + (size_t)row * N;
     double acc = 0.0;
+#pragma unroll 4
     for (int j = lane; j < N; j += 64) acc = fma(k[j], n[j], acc);
     acc = wave_sum(acc);
     if (lane == 0) U[row] = (U_ext ? U_ext[row] : 0.0) - V_gate - acc;
@@ -995,6 +996,7 @@ __global__ __launch_bounds__(256) void scf_potential_kernel(int N, const double*
     if (row >= N) return;
     const double* __restrict__ k = K + (size_t)row * N;
     double acc = 0.0;
+#pragma unroll 4
     for (int j = lane; j < N; j += 64) acc = fma(k[j], n[j], acc);
     acc = wave_sum(acc);
     if (lane == 0) {
