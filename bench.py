@@ -248,7 +248,10 @@ def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
     convergence at max |n_in - n_out| <= 1e-8.  The BF16 surrogate is trained
     first (not timed) on n_pairs exact NEGF (U, Q) pairs of seeded smooth
     potentials.  Timed: one batched run from n = 0, plain Broyden (fraction
-    0.1) against the surrogate-seeded, preconditioned run."""
+    0.1) against the run preconditioned by (I + c_t K)^-1 with the
+    capacitances c_t from the surrogate's tangent ("surrogate"), and against
+    the same run after 6 preconditioned steps on the surrogate charge
+    ("surrogate_seeded")."""
     import torch
     from electrostatic_amd import negf
     from electrostatic_amd.surrogate import Surrogate, negf_pairs, sample_potentials
@@ -279,7 +282,8 @@ def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
            "train_loss": loss}
     runs = {}
     variants = (("plain", lambda: scf.run(V)),
-                ("surrogate", lambda: scf.run_surrogate(V, seed_iters=6, refresh_iter=0)))
+                ("surrogate", lambda: scf.run_surrogate(V, seed_iters=0, refresh_iter=0)),
+                ("surrogate_seeded", lambda: scf.run_surrogate(V, seed_iters=6, refresh_iter=0)))
     for key, fn in variants:
         scf.set_input_charge(np.zeros((T, N)))
         fn()                      # warm-up: graph capture, path set-up
@@ -292,7 +296,7 @@ def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
         out[key] = {"s": dt, "bias_points_per_s": T / dt, "iterations_max": int(r[2].max()),
                     "iterations_mean": float(r[2].mean()), "batched_iterations": int(r[4])}
     U0 = runs["plain"][0]
-    for key in ("surrogate",):
+    for key in ("surrogate", "surrogate_seeded"):
         out[key]["max_abs_dU_vs_plain_V"] = float(np.abs(runs[key][0] - U0).max())
         out[key]["speedup_vs_plain"] = out["plain"]["s"] / out[key]["s"]
     out["capacitance_e_per_eV"] = [float(runs["surrogate"][5].min()), float(runs["surrogate"][5].max())]

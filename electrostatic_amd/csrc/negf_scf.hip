@@ -359,22 +359,23 @@ static void coulomb_kernel(const std::vector<double>& ptd, const negf_scf_params
 // all tubes, one batched NEGF solve, and one Broyden workgroup per tube.
 
 // Partial products Yp[z][t][i] = sum_{j in chunk z} A[i][j] X[t][j]
-// (A row-major N x N; X rows ldx apart; T rows) on v_mfma_f64_16x16x4f64,
-// chunk z = [z kchunk, (z + 1) kchunk).  A workgroup of 4 wavefronts computes
-// a 32 (i) x 32 (t) tile, one 16 x 16 quadrant per wavefront, over 32-wide
-// j blocks staged in LDS; the next block's global loads are issued before
-// the present block's MFMAs.  Lane l supplies A[i0 + l % 16][j + l / 16] and
-// X[t0 + l % 16][j + l / 16]; result register v of lane l is the entry
-// (i = l / 16 + 4 v, t = l % 16).  The caller sums the chunks in order.
+// (A row-major M x K, rows lda apart; X rows ldx apart; T rows) on
+// v_mfma_f64_16x16x4f64, chunk z = [z kchunk, (z + 1) kchunk); Yp[z] is
+// T x M.  A workgroup of 4 wavefronts computes a 32 (i) x 32 (t) tile, one
+// 16 x 16 quadrant per wavefront, over 32-wide j blocks staged in LDS; the
+// next block's global loads are issued before the present block's MFMAs.
+// Lane l supplies A[i0 + l % 16][j + l / 16] and X[t0 + l % 16][j + l / 16];
+// result register v of lane l is the entry (i = l / 16 + 4 v, t = l % 16).
+// The caller sums the chunks in order.
 typedef __attribute__((ext_vector_type(4))) double d4;
-__global__ __launch_bounds__(256) void mfma_dgemm_kernel(int N, int T, const double* __restrict__ A,
-                                                         const double* __restrict__ X, size_t ldx, int kchunk,
-                                                         double* __restrict__ Yp)
+__global__ __launch_bounds__(256) void mfma_dgemm_kernel(int M, int K, int T, const double* __restrict__ A,
+                                                         size_t lda, const double* __restrict__ X, size_t ldx,
+                                                         int kchunk, double* __restrict__ Yp)
 {
     __shared__ double As[32][33];
     __shared__ double Xs[32][33];
     const int i0 = blockIdx.x * 32, t0 = blockIdx.y * 32;
-    const int jb = blockIdx.z * kchunk, je = min(N, jb + kchunk);
+    const int jb = blockIdx.z * kchunk, je = min(K, jb + kchunk);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int qi = w >> 1, qt = w & 1;
     double ra[4], rx[4];
@@ -384,7 +385,7 @@ __global__ __launch_bounds__(256) void mfma_dgemm_kernel(int N, int T, const dou
             const int e = tid + 256 * q;
             const int r = e >> 5, c = e & 31;
             const int i = i0 + r, t = t0 + r, j = j0 + c;
-            ra[q] = (i < N && j < je) ? A[(size_t)i * N + j] : 0.0;
+            ra[q] = (i < M && j < je) ? A[(size_t)i * lda + j] : 0.0;
             rx[q] = (t < T && j < je) ? X[(size_t)t * ldx + j] : 0.0;
         }
     };
@@ -407,32 +408,33 @@ __global__ __launch_bounds__(256) void mfma_dgemm_kernel(int N, int T, const dou
         }
         __syncthreads();
     }
-    double* Y = Yp + (size_t)blockIdx.z * T * N;
+    double* Y = Yp + (size_t)blockIdx.z * T * M;
     const int t = t0 + qt * 16 + (lane & 15);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int i = i0 + qi * 16 + (lane >> 4) + 4 * v;
-        if (i < N && t < T) Y[(size_t)t * N + i] = acc[v];
+        if (i < M && t < T) Y[(size_t)t * M + i] = acc[v];
     }
 }
 
-// K-split of the GEMM: enough workgroups to cover the chip (about 1024),
-// chunks a multiple of 32 wide
-static int gemm_splits(int N, int T)
+// K-split of a GEMM with M output rows and inner length K: enough
+// workgroups to cover the chip (about 1024), chunks a multiple of 32 wide
+static int gemm_splits(int M, int K, int T)
 {
-    const int tiles = ((N + 31) / 32) * ((T + 31) / 32);
+    const int tiles = ((M + 31) / 32) * ((T + 31) / 32);
     int z = std::max(1, std::min(8, 1024 / std::max(tiles, 1)));
-    while (z > 1 && (N + 32 * z - 1) / (32 * z) < 4) --z;  // keep >= 4 blocks per chunk
+    while (z > 1 && (K + 32 * z - 1) / (32 * z) < 4) --z;  // keep >= 4 blocks per chunk
     return z;
 }
 
-static int gemm_chunk(int N, int z) { return ((N + z - 1) / z + 31) / 32 * 32; }
+static int gemm_chunk(int K, int z) { return ((K + z - 1) / z + 31) / 32 * 32; }
 
-static void launch_dgemm(int N, int T, int z, const double* A, const double* X, size_t ldx, double* Yp,
-                         hipStream_t st)
+// Yp[z] (T x M) = A (M x K, rows lda apart) X^T over chunk z of the inner index
+static void launch_dgemm(int M, int K, int T, int z, const double* A, size_t lda, const double* X, size_t ldx,
+                         double* Yp, hipStream_t st)
 {
-    dim3 g((N + 31) / 32, (T + 31) / 32, z);
-    hipLaunchKernelGGL(mfma_dgemm_kernel, g, dim3(256), 0, st, N, T, A, X, ldx, gemm_chunk(N, z), Yp);
+    dim3 g((M + 31) / 32, (T + 31) / 32, z);
+    hipLaunchKernelGGL(mfma_dgemm_kernel, g, dim3(256), 0, st, M, K, T, A, lda, X, ldx, gemm_chunk(K, z), Yp);
 }
 
 // Y = sum of the z partial products, in chunk order
@@ -520,21 +522,25 @@ __global__ void batch_residual_kernel(int N, int T, const double* __restrict__ n
     R[q] = n_in[(size_t)t * stride + i] - Q[q];
 }
 
-// eigen-space scaling of the preconditioner P_t = Z diag(1 / (1 + c_t lam)) Z^T
-__global__ void batch_eigscale_kernel(int N, int T, const double* __restrict__ lam, const double* __restrict__ c,
-                                      const double* __restrict__ parts, int nz, double* __restrict__ X)
+// Mode-space factor of the preconditioner's correction:
+// Y[t][k] = (sum of the z partial products) * c_t lam_k / (1 + c_t lam_k)
+__global__ void batch_modescale_kernel(int R, int T, const double* __restrict__ lam, const double* __restrict__ c,
+                                       const double* __restrict__ parts, int nz, double* __restrict__ Y)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long NT = (long long)N * T;
-    if (q >= NT) return;
-    const int t = (int)(q / N), i = (int)(q - (long long)t * N);
-    X[q] = sum_parts(parts, nz, NT, q) / (1.0 + c[t] * lam[i]);
+    const long long RT = (long long)R * T;
+    if (q >= RT) return;
+    const int t = (int)(q / R), k = (int)(q - (long long)t * R);
+    const double cl = c[t] * lam[k];
+    Y[q] = sum_parts(parts, nz, RT, q) * (cl / (1.0 + cl));
 }
 
-__global__ void batch_sum_parts_kernel(long long NT, const double* __restrict__ parts, int nz, double* __restrict__ X)
+// X = Rv - (sum of the z partial products)
+__global__ void batch_minus_parts_kernel(long long NT, const double* __restrict__ Rv,
+                                         const double* __restrict__ parts, int nz, double* __restrict__ X)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < NT) X[q] = sum_parts(parts, nz, NT, q);
+    if (q < NT) X[q] = Rv[q] - sum_parts(parts, nz, NT, q);
 }
 
 // c_t = max(0, mean_s J[t][s]) for the tubes in use: the site-averaged
@@ -599,7 +605,7 @@ class BatchScf {
         if ((rc = d_K.ensure(sizeof(double) * K.size()))) return rc;
         if ((rc = d_Uext.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_V.ensure(sizeof(double) * T))) return rc;
-        nz = gemm_splits(N, T);
+        nz = gemm_splits(N, N, T);
         if ((rc = d_KN.ensure(sizeof(double) * N * T * nz))) return rc;
         if ((rc = d_U.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_Q.ensure(sizeof(double) * N * T))) return rc;
@@ -825,25 +831,47 @@ class BatchScf {
     }
 
     // ---- surrogate-seeded, preconditioned iteration
+    // P_t = Z diag(1 / (1 + c_t lam)) Z^T = I - Z diag(c_t lam / (1 + c_t lam)) Z^T,
+    // applied with the modes of lam_k >= kPcModeCut * max lam only: a dropped
+    // mode's factor differs from 1 by c_t lam_k / (1 + c_t lam_k) < c_t
+    // kPcModeCut max lam, and P only sets the mixing, not the fixed point.
+    static constexpr double kPcModeCut = 1e-3;
     int set_preconditioner(const double* Z, const double* lam)
     {
         int rc;
-        const size_t NN = (size_t)N * N;
-        if ((rc = d_Z.ensure(sizeof(double) * NN))) return rc;
-        if ((rc = d_Zt.ensure(sizeof(double) * NN))) return rc;
-        if ((rc = d_lam.ensure(sizeof(double) * N))) return rc;
+        double lmax = 0.0;
+        for (int k = 0; k < N; ++k) lmax = std::max(lmax, lam[k]);
+        std::vector<int> keep;
+        for (int k = 0; k < N; ++k)
+            if (lam[k] > 0.0 && lam[k] >= kPcModeCut * lmax) keep.push_back(k);
+        R_pc = std::max<int>(1, (int)keep.size());
+        const size_t NR = (size_t)N * R_pc;
+        if ((rc = d_Z.ensure(sizeof(double) * NR))) return rc;
+        if ((rc = d_Zt.ensure(sizeof(double) * NR))) return rc;
+        if ((rc = d_lam.ensure(sizeof(double) * R_pc))) return rc;
         if ((rc = d_c.ensure(sizeof(double) * T))) return rc;
         if ((rc = d_R.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_Rh.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_PF.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_ones.ensure(sizeof(double) * N * T))) return rc;
         if ((rc = d_J.ensure(sizeof(double) * N * T))) return rc;
-        std::vector<double> zt(NN);
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) zt[(size_t)i * N + j] = Z[(size_t)j * N + i];
-        NEGF_HIP_TRY(hipMemcpy(d_Z.p, Z, sizeof(double) * NN, hipMemcpyHostToDevice));
-        NEGF_HIP_TRY(hipMemcpy(d_Zt.p, zt.data(), sizeof(double) * NN, hipMemcpyHostToDevice));
-        NEGF_HIP_TRY(hipMemcpy(d_lam.p, lam, sizeof(double) * N, hipMemcpyHostToDevice));
+        z_pc1 = gemm_splits(R_pc, N, T);
+        z_pc2 = gemm_splits(N, R_pc, T);
+        const size_t kn = std::max({(size_t)N * T * nz, (size_t)R_pc * T * z_pc1, (size_t)N * T * z_pc2});
+        if ((rc = d_KN.ensure(sizeof(double) * kn))) return rc;
+        // Zr: N x R (row i: the kept modes' entries at site i); Zrt: R x N
+        std::vector<double> zr(NR, 0.0), zrt(NR, 0.0), lr(R_pc, 0.0);
+        for (int q = 0; q < (int)keep.size(); ++q) {
+            const int k = keep[q];
+            lr[q] = lam[k];
+            for (int i = 0; i < N; ++i) {
+                zr[(size_t)i * R_pc + q] = Z[(size_t)i * N + k];
+                zrt[(size_t)q * N + i] = Z[(size_t)i * N + k];
+            }
+        }
+        NEGF_HIP_TRY(hipMemcpy(d_Z.p, zr.data(), sizeof(double) * NR, hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_Zt.p, zrt.data(), sizeof(double) * NR, hipMemcpyHostToDevice));
+        NEGF_HIP_TRY(hipMemcpy(d_lam.p, lr.data(), sizeof(double) * R_pc, hipMemcpyHostToDevice));
         if ((rc = fill_device(d_ones.as<double>(), (long long)N * T, 1.0, st))) return rc;
         if ((rc = fill_device(d_c.as<double>(), T, 0.0, st))) return rc;
         NEGF_HIP_TRY(hipStreamSynchronize(st));
@@ -854,7 +882,7 @@ class BatchScf {
     // U_t = U_ext,t - V_t - K n_in,t into the tubes and d_U
     int enqueue_potential()
     {
-        launch_dgemm(N, T, nz, d_K.as<double>(), base().n_in, state_stride, d_KN.as<double>(), st);
+        launch_dgemm(N, N, T, nz, d_K.as<double>(), (size_t)N, base().n_in, state_stride, d_KN.as<double>(), st);
         const long long NT = (long long)N * T;
         hipLaunchKernelGGL(batch_potential_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
                            d_Uext.as<double>(), d_V.as<double>(), d_KN.as<double>(), nz,
@@ -862,19 +890,22 @@ class BatchScf {
         NEGF_HIP_TRY(hipGetLastError());
         return NEGF_OK;
     }
-    // d_PF = P (n_in - Q) from the charge Q (T x N)
+    // d_PF = P (n_in - Q) from the charge Q (T x N): R = n_in - Q,
+    // Y = Zr^T R, Y_k *= c_t lam_k / (1 + c_t lam_k), PF = R - Zr Y
     int enqueue_precondition(const double* Q)
     {
-        const long long NT = (long long)N * T;
+        const long long NT = (long long)N * T, RT = (long long)R_pc * T;
         const dim3 g((unsigned)((NT + 255) / 256));
         hipLaunchKernelGGL(batch_residual_kernel, g, dim3(256), 0, st, N, T, base().n_in, state_stride, Q,
                            d_R.as<double>());
-        launch_dgemm(N, T, nz, d_Zt.as<double>(), d_R.as<double>(), (size_t)N, d_KN.as<double>(), st);
-        hipLaunchKernelGGL(batch_eigscale_kernel, g, dim3(256), 0, st, N, T, d_lam.as<double>(), d_c.as<double>(),
-                           d_KN.as<double>(), nz, d_Rh.as<double>());
-        launch_dgemm(N, T, nz, d_Z.as<double>(), d_Rh.as<double>(), (size_t)N, d_KN.as<double>(), st);
-        hipLaunchKernelGGL(batch_sum_parts_kernel, g, dim3(256), 0, st, NT, d_KN.as<double>(), nz,
-                           d_PF.as<double>());
+        launch_dgemm(R_pc, N, T, z_pc1, d_Zt.as<double>(), (size_t)N, d_R.as<double>(), (size_t)N,
+                     d_KN.as<double>(), st);
+        hipLaunchKernelGGL(batch_modescale_kernel, dim3((unsigned)((RT + 255) / 256)), dim3(256), 0, st, R_pc, T,
+                           d_lam.as<double>(), d_c.as<double>(), d_KN.as<double>(), z_pc1, d_Rh.as<double>());
+        launch_dgemm(N, R_pc, T, z_pc2, d_Z.as<double>(), (size_t)R_pc, d_Rh.as<double>(), (size_t)R_pc,
+                     d_KN.as<double>(), st);
+        hipLaunchKernelGGL(batch_minus_parts_kernel, g, dim3(256), 0, st, NT, d_R.as<double>(), d_KN.as<double>(),
+                           z_pc2, d_PF.as<double>());
         NEGF_HIP_TRY(hipGetLastError());
         return NEGF_OK;
     }
@@ -968,6 +999,7 @@ class BatchScf {
     std::vector<negf_tube*> tubes;
     int T = 0, N = 0, M = 0;
     int nz = 1;  // K-split of the GEMMs (partials in d_KN)
+    int R_pc = 0, z_pc1 = 1, z_pc2 = 1;   // preconditioner modes and its GEMMs' K-splits
     negf_scf_params p{};
     std::vector<double> K;
     std::vector<double*> hU, hUc;
