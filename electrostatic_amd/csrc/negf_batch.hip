@@ -412,6 +412,11 @@ int batch_enqueue(negf_batch* b, double* dQ, hipStream_t st, const int* tube_act
 }
 long long batch_generation(const negf_batch* b) { return b->b.generation; }
 int batch_poll_status(negf_batch* b) { return b->b.poll_status(); }
+void batch_status_words(const negf_batch* b, const int** eq, const int** neq)
+{
+    *eq = b->b.eq_plan.status_ptr();
+    *neq = b->b.n_neq > 0 ? b->b.neq_plan.status_ptr() : nullptr;
+}
 }  // namespace negf
 
 extern "C" {

@@ -143,6 +143,9 @@ long long batch_generation(const negf_batch* b);
 // Fold the plans' device status words into the batch's sticky root flag
 // (reported and cleared by negf_batch_status); synchronises.
 int batch_poll_status(negf_batch* b);
+// The device status words of the plans batch_enqueue launches (neq: nullptr
+// when the batch has no non-equilibrium plan); valid until the next rebuild.
+void batch_status_words(const negf_batch* b, const int** eq, const int** neq);
 int max_sites();
 
 }  // namespace negf

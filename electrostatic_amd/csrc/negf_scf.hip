@@ -499,17 +499,47 @@ __global__ __launch_bounds__(kBroydenThreads) void broyden_batch_kernel(int S, i
     }
 }
 
-// Keep the U and Q of tubes that converged at this iteration.
-__global__ void batch_keep_kernel(int N, int T, const int* __restrict__ keep, const double* __restrict__ U,
-                                  const double* __restrict__ Q, double* __restrict__ Uo, double* __restrict__ Qo)
+// After the Broyden updates (one workgroup per tube): the tubes' Broyden
+// outputs (max norm, l2 norm, |dF|^2, step-limit flag) and the plans' status
+// words go to the host's read-back record; an active tube whose max norm
+// reached max_norm keeps its U and Q (Uo, Qo) and leaves the active set, as
+// does one at its step limit.  The host applies the same tests to the same
+// values, so its bookkeeping follows without copies.
+__global__ __launch_bounds__(256) void batch_track_kernel(int N, const double* __restrict__ out, double max_norm,
+                                                          int* __restrict__ active, const double* __restrict__ U,
+                                                          const double* __restrict__ Q, double* __restrict__ Uo,
+                                                          double* __restrict__ Qo, double* __restrict__ back,
+                                                          int* __restrict__ back_status,
+                                                          const int* __restrict__ st_eq,
+                                                          const int* __restrict__ st_neq)
 {
-    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= (long long)N * T) return;
-    const int t = (int)(q / N);
-    if (keep[t]) {
-        Uo[q] = U[q];
-        Qo[q] = Q[q];
+    __shared__ int keep;
+    const int t = blockIdx.x;
+    if (threadIdx.x == 0) {
+        const double nrm = out[4 * t], lim = out[4 * t + 3];
+        for (int k = 0; k < 4; ++k) back[4 * t + k] = out[4 * t + k];
+        int kp = 0;
+        if (active[t]) {
+            if (lim != 0.0)
+                active[t] = 0;
+            else if (nrm <= max_norm)  // false for NaN: the host stops on it
+                kp = 1;
+        }
+        keep = kp;
+        if (t == 0) {
+            back_status[0] = *st_eq;
+            back_status[1] = st_neq ? *st_neq : 0;
+        }
+        __threadfence_system();
     }
+    __syncthreads();
+    if (!keep) return;
+    const size_t o = (size_t)t * N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        Uo[o + i] = U[o + i];
+        Qo[o + i] = Q[o + i];
+    }
+    if (threadIdx.x == 0) active[t] = 0;
 }
 
 // R[t][s] = n_in,t[s] - Q[t][s] (the reference's F = n_in - n_out)
@@ -580,6 +610,7 @@ class BatchScf {
             (void)hipStreamSynchronize(st);
             (void)hipStreamDestroy(st);
         }
+        if (h_back) (void)hipHostFree(h_back);
     }
     int init(negf_batch* b, negf_tube* const* tubes, int T_, const negf_scf_params& prm)
     {
@@ -618,8 +649,16 @@ class BatchScf {
         if ((rc = d_out.ensure(sizeof(double) * 4 * T))) return rc;
         if ((rc = d_steps.ensure(sizeof(int) * T))) return rc;
         if ((rc = d_active.ensure(sizeof(int) * T))) return rc;
-        if ((rc = d_keep.ensure(sizeof(int) * T))) return rc;
-        if ((rc = h_out.ensure(sizeof(double) * 4 * T))) return rc;
+        {
+            // read-back record: 4 doubles per tube, then 2 status ints
+            hipError_t eh = hipHostMalloc(&h_back, sizeof(double) * 4 * T + 2 * sizeof(int),
+                                          hipHostMallocMapped | hipHostMallocCoherent);
+            if (eh == hipSuccess) eh = hipHostGetDevicePointer(&d_back, h_back, 0);
+            if (eh != hipSuccess) {
+                set_error("mapped host buffer for the batched SCF read-back: %s", hipGetErrorString(eh));
+                return NEGF_EALLOC;
+            }
+        }
         NEGF_HIP_TRY(hipMemcpy(d_K.p, K.data(), sizeof(double) * K.size(), hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_ptrU.p, hU.data(), sizeof(double*) * T, hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_ptrUc.p, hUc.data(), sizeof(double*) * T, hipMemcpyHostToDevice));
@@ -660,7 +699,19 @@ class BatchScf {
                            p.Broyden_fraction, p.Broyden_norm_type, d_Q.as<double>(), nullptr, base(), state_stride,
                            hist_stride, d_steps.as<int>(), d_active.as<int>());
         NEGF_HIP_TRY(hipGetLastError());
-        NEGF_HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, sizeof(double) * 4 * T, hipMemcpyDeviceToHost, st));
+        return enqueue_track();
+    }
+    int enqueue_track()
+    {
+        const int* st_eq = nullptr;
+        const int* st_neq = nullptr;
+        batch_status_words(batch, &st_eq, &st_neq);
+        double* back = static_cast<double*>(d_back);
+        hipLaunchKernelGGL(batch_track_kernel, dim3(T), dim3(256), 0, st, N, d_out.as<const double>(),
+                           p.Broyden_max_norm, d_active.as<int>(), d_U.as<const double>(), d_Q.as<const double>(),
+                           d_Uo.as<double>(), d_Qo.as<double>(), back, reinterpret_cast<int*>(back + 4 * T), st_eq,
+                           st_neq);
+        NEGF_HIP_TRY(hipGetLastError());
         return NEGF_OK;
     }
     void drop_graph()
@@ -736,9 +787,9 @@ class BatchScf {
     int after_iteration(int it, LoopState& ls)
     {
         int rc;
-        if ((rc = batch_poll_status(batch))) return rc;
-        const double* o = h_out.as<double>();
-        bool any = false;
+        const volatile double* o = static_cast<const volatile double*>(h_back);
+        const volatile int* bs = reinterpret_cast<const volatile int*>(o + 4 * T);
+        if ((bs[0] | bs[1]) && (rc = batch_poll_status(batch))) return rc;
         for (int t = 0; t < T; ++t) {
             ls.keep[t] = 0;
             if (!ls.active[t]) continue;
@@ -748,27 +799,18 @@ class BatchScf {
                 --ls.remaining;
                 continue;
             }
-            if (!std::isfinite(o[4 * t])) {
+            const double nrm = o[4 * t];
+            if (!std::isfinite(nrm)) {
                 set_error("batched SCF: non-finite Broyden norm for tube %d at iteration %d", t, it + 1);
                 return NEGF_ENAN;
             }
-            if (o[4 * t] <= p.Broyden_max_norm) {
+            if (nrm <= p.Broyden_max_norm) {  // the track kernel kept U, Q and cleared the flag
                 ls.keep[t] = 1;
                 ls.active[t] = 0;
                 ls.done_it[t] = it + 1;
-                ls.fin[t] = o[4 * t];
+                ls.fin[t] = nrm;
                 --ls.remaining;
-                any = true;
             }
-        }
-        if (any) {
-            NEGF_HIP_TRY(hipMemcpyAsync(d_keep.p, ls.keep.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
-            const long long NT = (long long)N * T;
-            hipLaunchKernelGGL(batch_keep_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
-                               d_keep.as<int>(), d_U.as<double>(), d_Q.as<double>(), d_Uo.as<double>(),
-                               d_Qo.as<double>());
-            NEGF_HIP_TRY(hipGetLastError());
-            NEGF_HIP_TRY(hipMemcpyAsync(d_active.p, ls.active.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
         }
         return NEGF_OK;
     }
@@ -929,8 +971,7 @@ class BatchScf {
                            p.Broyden_fraction, p.Broyden_norm_type, d_Q.as<double>(), d_PF.as<double>(), base(),
                            state_stride, hist_stride, d_steps.as<int>(), d_active.as<int>());
         NEGF_HIP_TRY(hipGetLastError());
-        NEGF_HIP_TRY(hipMemcpyAsync(h_out.p, d_out.p, sizeof(double) * 4 * T, hipMemcpyDeviceToHost, st));
-        return NEGF_OK;
+        return enqueue_track();
     }
     // Seeded, preconditioned run: seed_iters preconditioned steps on the
     // surrogate charge (a fixed-point iteration of the surrogate SCF), then
@@ -1005,8 +1046,9 @@ class BatchScf {
     std::vector<double*> hU, hUc;
     size_t state_stride = 0, hist_stride = 0;
     DevBuf d_K, d_Uext, d_V, d_KN, d_U, d_Q, d_Uo, d_Qo, d_ptrU, d_ptrUc, d_state, d_hist, d_out, d_steps,
-        d_active, d_keep;
-    HostBuf h_out;
+        d_active;
+    void* h_back = nullptr;   // mapped read-back record (host address)
+    void* d_back = nullptr;   // the same memory as the device sees it
     bool has_ext = false, use_graph = true;
     hipStream_t st = nullptr;
     hipGraphExec_t gexec = nullptr;
