@@ -271,6 +271,10 @@ def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
     train_s = time.perf_counter() - t0
     del dU, dQ
     torch.cuda.empty_cache()
+    # segment length from the autotuner (both runs use it)
+    dQb = torch.zeros(T * N, dtype=torch.float64, device=dev)
+    seg_L, _ = batch.autotune(dQb.data_ptr())
+    del dQb
     sp = dict(eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
     scf = negf.BatchScf(batch, Broyden_fraction=0.1, Broyden_max_norm=1e-8, Broyden_threshold_maxstep=300, **sp)
     U_ext = np.stack([gaussian_bumps(N, seed=100 + t, amp=0.3) for t in range(T)])
@@ -279,7 +283,7 @@ def surrogate_leg(dev, n_pairs=20000, epochs=5, with_cpu=True):
     scf.set_surrogate(sg)
     out = {"workload": "128 (17,0) tubes x 2048 sites, 1 mode, dense 1/r coupling, gates 0..1 V, norm <= 1e-8",
            "pairs": n_pairs, "epochs": epochs, "pair_generation_s": gen_s, "training_s": train_s,
-           "train_loss": loss}
+           "train_loss": loss, "segment_length": seg_L}
     runs = {}
     variants = (("plain", lambda: scf.run(V)),
                 ("surrogate", lambda: scf.run_surrogate(V, seed_iters=0, refresh_iter=0)),
