@@ -491,7 +491,8 @@ def main():
                     help="also time the surrogate's forward and tangent passes alone")
     ap.add_argument("--write-diag-g", action="store_true",
                     help="also materialise diag G^R ([energy][site], 512 MiB)")
-    ap.add_argument("--only", choices=["batch", "block", "surrogate-kernels", "copy-calibration", "scf"],
+    ap.add_argument("--only", choices=["batch", "block", "surrogate-kernels", "copy-calibration", "scf",
+                                       "surrogate-scf"],
                     help="run one secondary leg alone and print its figures (for kernel traces and "
                          "counters; --segment-length then fixes the batch leg's segment length)")
     args = ap.parse_args()
@@ -507,7 +508,8 @@ def main():
                 "block": lambda: block_leg(with_cpu=False),
                 "surrogate-kernels": lambda: surrogate_kernels_leg(dev),
                 "copy-calibration": lambda: copy_calibration_leg(dev),
-                "scf": lambda: scf_sweep(with_cpu=False)}
+                "scf": lambda: scf_sweep(with_cpu=False),
+                "surrogate-scf": lambda: surrogate_leg(dev, with_cpu=False)}
         print(json.dumps({"only": args.only, "result": legs[args.only]()}), flush=True)
         return
 

@@ -877,7 +877,7 @@ class BatchScf {
     // applied with the modes of lam_k >= kPcModeCut * max lam only: a dropped
     // mode's factor differs from 1 by c_t lam_k / (1 + c_t lam_k) < c_t
     // kPcModeCut max lam, and P only sets the mixing, not the fixed point.
-    static constexpr double kPcModeCut = 1e-3;
+    static constexpr double kPcModeCut = 1e-2;
     int set_preconditioner(const double* Z, const double* lam)
     {
         int rc;

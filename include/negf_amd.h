@@ -551,7 +551,7 @@ int negf_surrogate_train(negf_surrogate *s, int nsamples, const double *d_U,
 int negf_batch_scf_get_kernel(const negf_batch_scf *s, double *K);
 /* Eigenbasis of K: Z (N x N row-major, column k = eigenvector k), lam[N].
  * The preconditioner I - Z_r diag(c lam / (1 + c lam)) Z_r^T keeps the modes
- * with lam >= 1e-3 max lam (the rest would change it by less than c lam). */
+ * with lam >= 1e-2 max lam (the rest would change it by less than c lam). */
 int negf_batch_scf_set_preconditioner(negf_batch_scf *s, const double *Z, const double *lam);
 /* The surrogate (same N) and its per-tube feature feat[T] (host, or NULL). */
 int negf_batch_scf_set_surrogate(negf_batch_scf *s, negf_surrogate *sg, const double *feat);
