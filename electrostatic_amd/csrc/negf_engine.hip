@@ -462,7 +462,7 @@ static int launch_rgf_finish(const RgfLayout& y, const WLaunch& lp, double* d_ou
 }
 
 int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st, const int* tube_active,
-                     double* d_site_out)
+                     double* d_site_out, bool finish)
 {
     if (nitems_ == 0) {
         set_error("RgfPlan::enqueue: plan not built");
@@ -543,6 +543,7 @@ int RgfPlan::enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipSt
         if (!ext) NEGF_HIP_TRY(hipEventRecord(ev_stop_[ev_used_], st));
         ++ev_used_;
     }
+    if (!finish && kind_ != KIND_DOS) return NEGF_OK;
     return launch_rgf_finish(y, lp, d_out, st, tube_active);
 }
 

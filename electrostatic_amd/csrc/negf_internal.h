@@ -85,8 +85,20 @@ class RgfPlan {
     // tube_active (device, indexed by RgfTube::ext, may be null): tubes with 0
     // are skipped and keep their previous outputs.  d_site_out (KIND_DOS,
     // optional, [chain][N]): each site's Gamma_L |G_n0|^2 + Gamma_R |G_n,N-1|^2.
+    // finish = false leaves out the last step, the fixed-order sum of the
+    // partial rows into d_out (KIND_EQ/NONEQ), for a caller that fuses it
+    // into its next kernel (single_tube_rows).
     int enqueue(double* d_out, double2* d_diagG, double* d_chain_out, hipStream_t st,
-                const int* tube_active = nullptr, double* d_site_out = nullptr);
+                const int* tube_active = nullptr, double* d_site_out = nullptr, bool finish = true);
+    // A one-tube plan's partial rows: row r of site n at part[r N + n], r <
+    // nrows, summed in order r = 0, 1, ... from 0.0; false for other plans.
+    bool single_tube_rows(const double** part, int* nrows) const
+    {
+        if (ntubes_ != 1 || kind_ == KIND_DOS) return false;
+        *part = d_part_.as<const double>();
+        *nrows = max_rows_;
+        return true;
+    }
     // Read and clear the device status word (synchronises the stream).
     int check_status(hipStream_t st, int* status);
     // The device status word itself (an int), for a copy enqueued by the caller.

@@ -44,9 +44,11 @@ class Tube {
     int solve(int iter, double* dQ, hipStream_t st);
     // The launches of a solve whose prepare() has run, all on stream st and
     // nothing on the host (capturable).  combine = false leaves out the last
-    // step, Q = (eq + noneq) - rho0 (d_eq, d_neq, d_rho0), for a caller
-    // that fuses it into its next kernel; dQ is then unused.
+    // two steps, the equilibrium plan's row sum into d_eq (eq_rows) and
+    // Q = (eq + noneq) - rho0 (d_eq, d_neq, d_rho0), for a caller that fuses
+    // both into its next kernel; dQ is then unused.
     int enqueue_solve(double* dQ, hipStream_t st, bool combine = true);
+    bool eq_rows(const double** part, int* nrows) const { return eq_plan.single_tube_rows(part, nrows); }
     // Device status words of the plans enqueue_solve() launches (nullptr for
     // a plan it does not launch); nonzero after a launch: check_status().
     const int* eq_status() const { return eq_plan.status_ptr(); }

@@ -623,7 +623,7 @@ int Tube::solve(int iter, double* dQ, hipStream_t st)
 int Tube::enqueue_solve(double* dQ, hipStream_t st, bool combine)
 {
     int rc;
-    if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st))) return rc;
+    if ((rc = eq_plan.enqueue(d_eq.as<double>(), nullptr, nullptr, st, nullptr, nullptr, combine))) return rc;
     const bool have_neq = L.noneq && n_neq > 0;
     if (have_neq) {
         neq_zero = false;

@@ -1090,13 +1090,17 @@ static void scf_enqueue_potential(Scf& s, hipStream_t q)
                        s.d_U.as<double>(), t.d_U.as<double>(), t.d_Uc.as<double>());
 }
 
-// The tube's charge Q = (eq + noneq) - rho0 (as combine_charge_kernel)
-// into n_out, then one Broyden update with the step counter on the device
-// (the host checks the step limit before each launch), then the norms and
-// the plans' status words to the host's read-back record.  Each thread
-// forms the Q entries that it reads first in the update.
+// With part: the tube's equilibrium charge from the plan's partial rows
+// (rows r = 0..R-1 of site s at part[r S + s], summed in order from 0.0 as
+// the plan's own row sum of at most 16 rows does) into eq, its charge
+// Q = (eq + noneq) - rho0 (as combine_charge_kernel) into n_out; without,
+// n_out holds Q already.  Then one Broyden update with the step counter on
+// the device (the host checks the step limit before each launch), then the
+// norms and the plans' status words to the host's read-back record.  Each
+// thread forms the Q entries that it reads first in the update.
 __global__ __launch_bounds__(kBroydenThreads) void broyden_scf_kernel(int S, double fraction, int norm_type,
-                                                                        const double* __restrict__ eq,
+                                                                        const double* __restrict__ part, int R,
+                                                                        double* __restrict__ eq,
                                                                         const double* __restrict__ neq,
                                                                         const double* __restrict__ rho0,
                                                                         double* n_out, BroydenDev b,
@@ -1105,7 +1109,15 @@ __global__ __launch_bounds__(kBroydenThreads) void broyden_scf_kernel(int S, dou
                                                                         const int* __restrict__ st_neq,
                                                                         ScfBack* __restrict__ back)
 {
-    for (int s = threadIdx.x; s < S; s += kBroydenThreads) n_out[s] = (eq[s] + (neq ? neq[s] : 0.0)) - rho0[s];
+    if (part) {
+        for (int s = threadIdx.x; s < S; s += kBroydenThreads) {
+            double e = 0.0;
+#pragma unroll 4
+            for (int r = 0; r < R; ++r) e += part[(size_t)r * S + s];
+            eq[s] = e;
+            n_out[s] = (e + (neq ? neq[s] : 0.0)) - rho0[s];
+        }
+    }
     const int m = *step - 1;
     broyden_update<false>(S, m, fraction, norm_type, n_out, nullptr, b);
     if (threadIdx.x == 0) {   // b.out[0..1] were written by this thread
@@ -1124,12 +1136,19 @@ static int scf_enqueue_iteration(Scf& s, hipStream_t q)
     Tube& t = *s.tube;
     scf_enqueue_potential(s, q);
     NEGF_HIP_TRY(hipGetLastError());
-    int rc = t.enqueue_solve(nullptr, q, false);
+    // the row sum and the charge go into the Broyden kernel when the plan's
+    // own row sum is the in-order one (at most 16 rows)
+    const double* part = nullptr;
+    int R = 0;
+    const bool fuse = t.eq_rows(&part, &R) && R <= 16;
+    if (!fuse) part = nullptr;
+    int rc = t.enqueue_solve(s.d_Q.as<double>(), q, !fuse);
     if (rc) return rc;
     hipLaunchKernelGGL(broyden_scf_kernel, dim3(1), dim3(kBroydenThreads), 0, q, s.br.S, s.br.fraction,
-                       s.br.norm_type, t.d_eq.as<const double>(), t.neq_status() ? t.d_neq.as<const double>() : nullptr,
-                       t.d_rho0.as<const double>(), s.d_Q.as<double>(), s.br.dev(), s.d_step.as<int>(),
-                       t.eq_status(), t.neq_status(), static_cast<ScfBack*>(s.d_back));
+                       s.br.norm_type, part, R, t.d_eq.as<double>(),
+                       t.neq_status() ? t.d_neq.as<const double>() : nullptr, t.d_rho0.as<const double>(),
+                       s.d_Q.as<double>(), s.br.dev(), s.d_step.as<int>(), t.eq_status(), t.neq_status(),
+                       static_cast<ScfBack*>(s.d_back));
     NEGF_HIP_TRY(hipGetLastError());
     return NEGF_OK;
 }

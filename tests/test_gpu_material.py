@@ -256,18 +256,21 @@ def test_scf_nonequilibrium_bias_point():
     assert np.abs(U - Uo).max() < U_TOL
 
 
-@pytest.mark.parametrize("mu,adaptive", [((-1.0, -1.0), 0), ((-1.0, -1.1), 1)])
-def test_scf_graph_replay_matches_enqueued_iterations(mu, adaptive):
+@pytest.mark.parametrize("mu,adaptive,eq_pts", [((-1.0, -1.0), 0, (30, 30, 64)), ((-1.0, -1.1), 1, (30, 30, 64)),
+                                                 ((-1.0, -1.0), 0, (400, 400, 400))])
+def test_scf_graph_replay_matches_enqueued_iterations(mu, adaptive, eq_pts):
     """The graph-replayed iteration (the default) and the same launches
     enqueued one by one give the same U, Q, iteration counts and norms bit
     for bit over a short sweep in which the external potential is switched
     on after the first bias point.  With adaptive limits (every 3rd
     iteration) under bias the paths are rebuilt inside the loop, so the graph
-    is recaptured there."""
+    is recaptured there.  The replay sums the equilibrium plan's partial rows
+    inside its Broyden kernel when there are at most 16 (143 contour chains:
+    3 rows); 1219 chains (20 rows) keep the plan's own row sum."""
     extra = dict(flag_adaptive_integration_limits=1, integrand_correction_interval=3) if adaptive else {}
     runs = []
     for graph in (True, False):
-        tube, _ = pair(num_unitcells=50, mu=mu, **extra)
+        tube, _ = pair(num_unitcells=50, mu=mu, eq_pts=eq_pts, **extra)
         scf = negf.Scf(tube, Broyden_fraction=0.1, Broyden_max_norm=1e-6, Broyden_threshold_maxstep=200,
                        eps_r=3.9, R_ring_nm=0.665, R_image_nm=10.0)
         scf.use_graph(graph)
