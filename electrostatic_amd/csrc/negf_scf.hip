@@ -592,13 +592,14 @@ __global__ __launch_bounds__(256) void batch_rowmean_kernel(int N, const double*
 }
 
 // n_in,t -= P R_t (a preconditioned step without history, for the seed)
+// One seed step n_in -= beta P (n_in - Q_surrogate)
 __global__ void batch_seed_step_kernel(int N, int T, double* __restrict__ n_in, size_t stride,
-                                       const double* __restrict__ PR)
+                                       const double* __restrict__ PR, double beta)
 {
     const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= (long long)N * T) return;
     const int t = (int)(q / N), i = (int)(q - (long long)t * N);
-    n_in[(size_t)t * stride + i] -= PR[q];
+    n_in[(size_t)t * stride + i] -= beta * PR[q];
 }
 
 class BatchScf {
@@ -973,6 +974,11 @@ class BatchScf {
         NEGF_HIP_TRY(hipGetLastError());
         return enqueue_track();
     }
+    // Seed steps are half steps: with full steps the scalar capacitance
+    // overshoots on the high-gate tubes, whose seed then oscillates (128
+    // tubes x 2048 sites: 3 full steps leave a tube unconverged after 300
+    // exact iterations; 6 half steps need 23 against 29 after 6 full ones).
+    static constexpr double kSeedStep = 0.5;
     // Seeded, preconditioned run: seed_iters preconditioned steps on the
     // surrogate charge (a fixed-point iteration of the surrogate SCF), then
     // the Broyden loop on the exact NEGF charge with the mixing term P F.  The
@@ -995,7 +1001,7 @@ class BatchScf {
             if ((rc = sg->output(T, d_Q.as<double>(), st))) return rc;
             if ((rc = enqueue_precondition(d_Q.as<double>()))) return rc;
             hipLaunchKernelGGL(batch_seed_step_kernel, dim3((unsigned)((NT + 255) / 256)), dim3(256), 0, st, N, T,
-                               base().n_in, state_stride, d_PF.as<double>());
+                               base().n_in, state_stride, d_PF.as<double>(), kSeedStep);
             NEGF_HIP_TRY(hipGetLastError());
             if ((rc = enqueue_potential())) return rc;
         }

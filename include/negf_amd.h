@@ -556,8 +556,9 @@ int negf_batch_scf_set_preconditioner(negf_batch_scf *s, const double *Z, const 
 /* The surrogate (same N) and its per-tube feature feat[T] (host, or NULL). */
 int negf_batch_scf_set_surrogate(negf_batch_scf *s, negf_surrogate *sg, const double *feat);
 /* Run all tubes to Broyden_Norm <= Broyden_max_norm from the present input
- * charge: seed_iters preconditioned fixed-point steps on the surrogate
- * charge first (the seed), then Broyden with P_t F on the exact NEGF charge.
+ * charge: seed_iters preconditioned fixed-point half steps on the surrogate
+ * charge first (the seed; n_in -= P_t (n_in - Q_surrogate) / 2), then
+ * Broyden with P_t F on the exact NEGF charge.
  * c_t comes from the surrogate at the start, half way through the seed, at
  * the seed, and at iteration refresh_iter of the exact loop (fresh history,
  * n_in kept; <= 0: never).  Outputs as negf_batch_scf_run; capacitance [T]
