@@ -199,6 +199,21 @@ __global__ __launch_bounds__(kMsThreads) void big_broyden_pass3(BigBroyden b, do
 
 __global__ void big_broyden_first_step(int* step) { *step = 1; }
 
+// The iteration's norms (coef[0..1]) and the batch plans' status words into
+// the host's mapped, coherent read-back record: no copy in the iteration.
+__global__ void multi_readback_kernel(const double* __restrict__ coef, const int* __restrict__ st_eq,
+                                      const int* __restrict__ st_neq, double* __restrict__ back)
+{
+    if (threadIdx.x == 0) {
+        back[0] = coef[0];
+        back[1] = coef[1];
+        int* bs = reinterpret_cast<int*>(back + 2);
+        bs[0] = *st_eq;
+        bs[1] = st_neq ? *st_neq : 0;
+        __threadfence_system();
+    }
+}
+
 // Electrostatic stand-in: U_i = U_ext,i - V_t - sum_j K(i, j) n_j within each
 // tube, K(i, j) = kt[i % 2][i - j + D].  The (rows x columns) plane of each
 // tube is cut into tiles of MS_ROWS x MS_TILE; a workgroup sums one tile's
@@ -303,6 +318,7 @@ class MultiScf {
             (void)hipStreamSynchronize(st);
             (void)hipStreamDestroy(st);
         }
+        if (h_back) (void)hipHostFree(h_back);
     }
     int init(negf_batch* b, negf_tube* const* tb, int T_, const negf_scf_params& prm)
     {
@@ -383,7 +399,16 @@ class MultiScf {
         if ((rc = d_part.ensure(sizeof(double) * kMsBlocks * (4 + 2 * (size_t)M)))) return rc;
         if ((rc = d_coef.ensure(sizeof(double) * (6 + 2 * (size_t)M)))) return rc;
         if ((rc = d_step.ensure(sizeof(int)))) return rc;
-        if ((rc = h_out.ensure(sizeof(double) * 4))) return rc;
+        {
+            // read-back record: 2 doubles (norms), then 2 status ints
+            hipError_t eh = hipHostMalloc(&h_back, 2 * sizeof(double) + 2 * sizeof(int),
+                                          hipHostMallocMapped | hipHostMallocCoherent);
+            if (eh == hipSuccess) eh = hipHostGetDevicePointer(&d_back, h_back, 0);
+            if (eh != hipSuccess) {
+                set_error("mapped host buffer for the multi-tube SCF read-back: %s", hipGetErrorString(eh));
+                return NEGF_EALLOC;
+            }
+        }
         NEGF_HIP_TRY(hipMemcpy(d_kt.p, kt.data(), sizeof(double) * kt.size(), hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_off.p, off.data(), sizeof(long long) * (T + 1), hipMemcpyHostToDevice));
         NEGF_HIP_TRY(hipMemcpy(d_tiles.p, tiles.data(), sizeof(MsTile) * tiles.size(), hipMemcpyHostToDevice));
@@ -451,8 +476,13 @@ class MultiScf {
         hipLaunchKernelGGL(big_broyden_pass2, dim3(1), dim3(kMsThreads), 0, st, bro(), nb);
         hipLaunchKernelGGL(big_broyden_pass3, dim3(nb), dim3(kMsThreads), 0, st, bro(), p.Broyden_fraction);
         NEGF_HIP_TRY(hipGetLastError());
-        // Broyden_Norm and the l2 norm to the host
-        NEGF_HIP_TRY(hipMemcpyAsync(h_out.p, d_coef.p, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+        // Broyden_Norm, the l2 norm and the plans' status words to the host
+        const int* st_eq = nullptr;
+        const int* st_neq = nullptr;
+        batch_status_words(batch, &st_eq, &st_neq);
+        hipLaunchKernelGGL(multi_readback_kernel, dim3(1), dim3(64), 0, st, d_coef.as<const double>(), st_eq, st_neq,
+                           static_cast<double*>(d_back));
+        NEGF_HIP_TRY(hipGetLastError());
         return NEGF_OK;
     }
     void drop_graph()
@@ -534,8 +564,10 @@ class MultiScf {
             float ms = 0.f;
             NEGF_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
             t_ms += ms;
-            if ((rc = batch_poll_status(batch))) break;
-            norm = h_out.as<double>()[0];
+            const volatile double* back = static_cast<const volatile double*>(h_back);
+            const volatile int* bs = reinterpret_cast<const volatile int*>(back + 2);
+            if ((bs[0] | bs[1]) && (rc = batch_poll_status(batch))) break;
+            norm = back[0];
             ++it;
             if (!std::isfinite(norm)) {
                 set_error("multi-tube SCF: non-finite Broyden norm at iteration %d", it);
@@ -565,7 +597,8 @@ class MultiScf {
     long long S = 0;
     negf_scf_params p{};
     DevBuf d_kt, d_off, d_tiles, d_rows, d_partial, d_ptrU, d_ptrUc, d_Uext, d_V, d_U, d_Q, d_state, d_hist, d_part, d_coef, d_step;
-    HostBuf h_out;
+    void* h_back = nullptr;   // mapped read-back record (host address)
+    void* d_back = nullptr;   // the same memory as the device sees it
     bool use_graph = true;
     long long gen = -1;
     hipStream_t st = nullptr;
