@@ -776,9 +776,10 @@ class BatchScf {
         return false;
     }
 
-    // After the iteration's synchronisation: fold the plans' status into the
-    // batch's sticky root flag, stop on a non-finite norm, retire converged
-    // tubes (their U and Q are kept).
+    // After the iteration's synchronisation, from the read-back record that
+    // the track kernel wrote: fold nonzero plan status words into the batch's
+    // sticky root flag, stop on a non-finite norm, and mirror the device's
+    // retirement of converged tubes (which already kept their U and Q).
     struct LoopState {
         std::vector<int> active, keep, done_it;
         std::vector<double> fin;
